@@ -1,0 +1,159 @@
+"""Throughput of the hydro-cal timestep on one MI355X.
+
+Workload: the 207,234-cell floodplain mesh (20 km x 8 km, meandering channel,
+wet/dry fronts), DT = 0.5 s, fp64 (headline) and fp32, synthetic seeded data.
+A "step" is one full timestep (flux of every slot + update of every cell).
+
+Prints ONE JSON line:
+  value       cell-steps/s of the timed fp64 steps (state resident in HBM)
+  roofline    the dominant kernel's algorithmic bytes / its launch duration
+              (HIP events on the launch stream), against 8 TB/s
+  cpu_baseline the oracle (C++ restatement of the reference kernels) timed
+              on a bounded sample of the same workload, 1 host thread
+
+Usage: python bench.py [--gpus 1] [--steps 900] [--warmup 20] [--mode persistent]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hydro-bench_amd"))
+sys.path.insert(0, ROOT)
+
+# Algorithmic bytes per cell-step at 4 slots per cell (reads: state H U V Z,
+# statics ZBC AREA FNC, per slot neighbour id 4 B + KLAS 1 B + COSF SINF SIDE;
+# writes: H U V Z W):  24 s + 20 bytes.
+BYTES_PER_CELL_STEP = {"f64": 24 * 8 + 20, "f32": 24 * 4 + 20}
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "floodplain_207234"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=900)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--mode", default="persistent", choices=["persistent", "graph", "fused", "two_kernel"])
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    return ap.parse_args()
+
+
+def time_gpu(host, mode, steps, warmup, block):
+    import torch
+
+    import hydro_amd
+    from hydro_amd.device import DeviceMesh
+
+    tune = hydro_amd.default_tune(block=block)
+    dm = DeviceMesh(host, device=0, tune=tune)
+    init = host.state()
+    if warmup:
+        dm.step(min(warmup, host.max_steps), mode)
+    torch.cuda.synchronize()
+    dm.load_state(init)
+    dm.set_time(0, 1)
+    dm.step(0, mode)   # re-gather outside the timed region
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    dm.step(steps, mode)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    launches = {"persistent": 1, "graph": steps, "fused": steps, "two_kernel": 2 * steps}[mode]
+    st = dm.fetch("H")
+    dm.close()
+    return ms, wall, launches, st
+
+
+def cpu_baseline(host, budget_s):
+    from oracle import hydro_oracle
+
+    r = hydro_oracle.OracleRun(host)
+    t0 = time.perf_counter()
+    r.step(1)
+    one = time.perf_counter() - t0
+    n = max(1, min(host.max_steps - 1, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    r.step(n)
+    dt = time.perf_counter() - t0
+    return host.CELL * n / dt, n, dt
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the timestep is not sharded "
+              f"in this build", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import hydro_amd
+
+    host64 = hydro_amd.make_config(WORKLOAD, dtype=np.float64)
+    steps = min(a.steps, host64.max_steps)
+    ms, wall, launches, st = time_gpu(host64, a.mode, steps, a.warmup, a.block)
+    if not np.all(np.isfinite(st["H"])):
+        raise SystemExit("non-finite state after the timed steps")
+    C = host64.CELL
+    value = C * steps / (ms / 1e3)
+    per_launch_ms = ms / launches
+    steps_per_launch = steps / launches if a.mode != "two_kernel" else 1.0
+    bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C * (steps if a.mode == "persistent" else 1)
+    if a.mode == "two_kernel":
+        bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C / 2.0
+    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    kernel = {"persistent": "k_persistent<double,0>", "graph": "k_fused_step_tick<double,0>",
+              "fused": "k_fused_step<double,0>", "two_kernel": "k_calculate_flux+k_update_cell"}[a.mode]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{a.mode}_{WORKLOAD}_f64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out = {
+        "metric": "cell-steps/sec",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 207,234-cell floodplain mesh, DT=0.5 s)",
+        "config": {"workload": WORKLOAD, "cells": C, "edges": host64.n_edges, "dt": host64.DT,
+                   "mode": a.mode, "block": a.block, "variant": "A"},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_cell_step": BYTES_PER_CELL_STEP["f64"], "launches": launches,
+                     "avg_launch_ms": per_launch_ms, "steps_per_launch": steps_per_launch},
+        "wall_s": wall,
+    }
+    if not a.no_fp32:
+        host32 = hydro_amd.make_config(WORKLOAD, dtype=np.float32)
+        ms32, _, l32, _ = time_gpu(host32, a.mode, steps, a.warmup, a.block)
+        v32 = C * steps / (ms32 / 1e3)
+        out["fp32"] = {"value": v32, "ms_per_step": ms32 / steps,
+                       "roofline_frac": v32 * BYTES_PER_CELL_STEP["f32"] / 1e9 / HBM_PEAK_GBS}
+    if not a.no_cpu:
+        v, n, dt = cpu_baseline(host64, a.cpu_seconds)
+        out["cpu_baseline"] = {"value": v, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle (C++ restatement of the reference kernels, g++ -O2), "
+                                         f"{n} fp64 steps of the same 207,234-cell mesh in {dt:.1f} s, 1 thread"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,733 @@
+// api.cpp -- the C ABI declared in include/hydro.h.
+#include "hydro.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "prep.hpp"
+
+using namespace hydro;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(...)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (__VA_ARGS__);                                                          \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(HYDRO_ERR_HIP, "%s failed: %s (%s:%d)", #__VA_ARGS__,                   \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    } while (0)
+
+// Device buffer owned by a mesh handle.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t n) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = n;
+        if (n == 0) return HYDRO_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) return fail(HYDRO_ERR_NOMEM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+        return HYDRO_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+struct GraphKey {
+    int mode, parity, chunk;
+    bool operator<(const GraphKey &o) const {
+        if (mode != o.mode) return mode < o.mode;
+        if (parity != o.parity) return parity < o.parity;
+        return chunk < o.chunk;
+    }
+};
+
+struct GraphExec {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~GraphExec() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+bool is_device_ptr(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified;
+}
+
+}  // namespace
+
+struct hydro_mesh {
+    hydro_mesh_desc d{};
+    int spd = 0;          // steps_per_day = (int)(MDT / DT)
+    int day = 0, kt = 1;  // time cursor: the next step runs at (kt, day)
+    int CELL = 0;
+    int rsize = 8;
+    hydro_tune tune{};
+    int block_two = 256;
+
+    // kinds present (from KLAS at create time)
+    bool has_kind[16] = {};
+
+    // reordered layout
+    bool prepared = false;
+    int cur = 0;  // state record buffer holding the current state
+    bool ext_valid = true, int_valid = false;
+    DevBuf st[2], W, ZBC, AREA, FNC, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig;
+    DevBuf tick, barrier, perr, ext_dev;
+    bool need_zb1 = false, need_slcs = false;
+    hipStream_t cap_stream = nullptr;
+    std::map<GraphKey, std::unique_ptr<GraphExec>> graphs;
+
+    // profiling
+    bool profiling = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    int last_launches = 0, last_steps = 0;
+
+    ~hydro_mesh() {
+        graphs.clear();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+namespace {
+
+template <class R> ExtView<R> ext_view(const hydro_mesh *m) {
+    const hydro_mesh_desc &d = m->d;
+    ExtView<R> v;
+    v.CELL = d.CELL;
+    v.NDAYS = d.NDAYS;
+    v.NHQ = d.NHQ;
+    v.S.DT = (R)d.DT;
+    v.S.HM1 = (R)d.HM1;
+    v.S.HM2 = (R)d.HM2;
+    v.S.JL = (R)d.JL;
+    v.H = (R *)d.H; v.U = (R *)d.U; v.V = (R *)d.V; v.Z = (R *)d.Z; v.W = (R *)d.W;
+    v.ZBC = (const R *)d.ZBC; v.ZB1 = (const R *)d.ZB1; v.AREA = (const R *)d.AREA;
+    v.FNC = (const R *)d.FNC; v.BF = (const R *)d.BoundaryFeature; v.NHQ1 = d.NHQ1;
+    v.ZW = (const R *)d.ZW; v.QW = (const R *)d.QW; v.QT = (const R *)d.QT; v.DQT = (const R *)d.DQT;
+    v.ZT = (const R *)d.ZT; v.DZT = (const R *)d.DZT;
+    v.NAC = d.NAC; v.KLAS = d.KLAS;
+    v.SIDE = (const R *)d.SIDE; v.COSF = (const R *)d.COSF; v.SINF = (const R *)d.SINF;
+    v.SLCOS = (const R *)d.SLCOS; v.SLSIN = (const R *)d.SLSIN;
+    v.F0 = (R *)d.FLUX0; v.F1 = (R *)d.FLUX1; v.F2 = (R *)d.FLUX2; v.F3 = (R *)d.FLUX3;
+    return v;
+}
+
+template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
+    FusedView<R> f;
+    f.C = m->CELL;
+    f.S.DT = (R)m->d.DT;
+    f.S.HM1 = (R)m->d.HM1;
+    f.S.HM2 = (R)m->d.HM2;
+    f.S.JL = (R)m->d.JL;
+    f.ext = m->ext_dev.as<ExtView<R>>();
+    f.Sin = m->st[in].as<State<R>>();
+    f.Sout = m->st[in ^ 1].as<State<R>>();
+    f.W = m->W.as<R>();
+    f.ZBC = m->ZBC.as<R>();
+    f.AREA = m->AREA.as<R>();
+    f.FNC = m->FNC.as<R>();
+    f.ZB1 = m->need_zb1 ? m->ZB1.as<R>() : nullptr;
+    f.nbr = m->nbr.as<int32_t>();
+    f.meta = m->meta.as<uint8_t>();
+    f.COSF = m->COSF.as<R>();
+    f.SINF = m->SINF.as<R>();
+    f.SIDE = m->SIDE.as<R>();
+    f.SLCOS = m->need_slcs ? m->SLCOS.as<R>() : nullptr;
+    f.SLSIN = m->need_slcs ? m->SLSIN.as<R>() : nullptr;
+    f.orig = m->orig.as<int32_t>();
+    f.materialize = m->tune.materialize_flux;
+    return f;
+}
+
+template <class T> int d2h(std::vector<T> &dst, const void *src, size_t n) {
+    dst.resize(n);
+    if (n == 0) return HYDRO_OK;
+    HIP_TRY(hipMemcpy(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+    return HYDRO_OK;
+}
+
+template <class T> int h2d(DevBuf &b, const std::vector<T> &src) {
+    int rc = b.alloc(src.size() * sizeof(T));
+    if (rc) return rc;
+    if (!src.empty()) HIP_TRY(hipMemcpy(b.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return HYDRO_OK;
+}
+
+template <class R> int prepare_t(hydro_mesh *m) {
+    const hydro_mesh_desc &d = m->d;
+    const int C = d.CELL;
+    const size_t S4 = 4 * (size_t)C;
+    std::vector<int32_t> NAC, KLAS;
+    std::vector<R> ZBC, ZB1, AREA, FNC, SIDE, COSF, SINF, SLCOS, SLSIN;
+    int rc;
+    if ((rc = d2h(NAC, d.NAC, S4)) || (rc = d2h(KLAS, d.KLAS, S4)) || (rc = d2h(ZBC, d.ZBC, C)) ||
+        (rc = d2h(ZB1, d.ZB1, C)) || (rc = d2h(AREA, d.AREA, C)) || (rc = d2h(FNC, d.FNC, C)) ||
+        (rc = d2h(SIDE, d.SIDE, S4)) || (rc = d2h(COSF, d.COSF, S4)) || (rc = d2h(SINF, d.SINF, S4)) ||
+        (rc = d2h(SLCOS, d.SLCOS, S4)) || (rc = d2h(SLSIN, d.SLSIN, S4)))
+        return rc;
+
+    std::vector<int32_t> perm;
+    if (m->tune.reorder == HYDRO_REORDER_RCM) {
+        perm = rcm_order(C, NAC.data());
+    } else if (m->tune.reorder == HYDRO_REORDER_NONE) {
+        perm.resize(C);
+        for (int i = 0; i < C; i++) perm[i] = i;
+    } else {
+        return fail(HYDRO_ERR_UNSUPPORTED, "reorder %d needs cell coordinates; use RCM or NONE",
+                    m->tune.reorder);
+    }
+    std::vector<int32_t> inv(C);
+    for (int p = 0; p < C; p++) inv[perm[p]] = p;
+
+    // Are ZB1 and SLCOS/SLSIN exactly what the kernel would recompute?
+    const R HM1 = (R)d.HM1;
+    bool zb1_ok = true, slcs_ok = true;
+    for (int i = 0; i < C && zb1_ok; i++) {
+        const R z = ZBC[i] + HM1;
+        zb1_ok = std::memcmp(&ZB1[i], &z, sizeof(R)) == 0;
+    }
+    for (size_t s = 0; s < S4 && slcs_ok; s++) {
+        R a = SIDE[s] * COSF[s], b = SIDE[s] * SINF[s];
+        slcs_ok = std::memcmp(&SLCOS[s], &a, sizeof(R)) == 0 && std::memcmp(&SLSIN[s], &b, sizeof(R)) == 0;
+    }
+    m->need_zb1 = !zb1_ok;
+    m->need_slcs = !slcs_ok;
+
+    std::vector<R> zbc_n(C), area_n(C), fnc_n(C), zb1_n(m->need_zb1 ? C : 0);
+    std::vector<int32_t> nbr_n(S4), orig_n(C);
+    std::vector<uint8_t> meta_n(S4);
+    std::vector<R> cos_n(S4), sin_n(S4), side_n(S4), slc_n(m->need_slcs ? S4 : 0), sls_n(m->need_slcs ? S4 : 0);
+    for (int p = 0; p < C; p++) {
+        const int o = perm[p];
+        orig_n[p] = o;
+        zbc_n[p] = ZBC[o];
+        area_n[p] = AREA[o];
+        fnc_n[p] = FNC[o];
+        if (m->need_zb1) zb1_n[p] = ZB1[o];
+        for (int j = 0; j < 4; j++) {
+            const size_t so = 4 * (size_t)o + j, sn = (size_t)j * C + p;
+            const int NC = NAC[so] - 1;
+            const bool prim = (o < NC);   // decided on ORIGINAL ids (F:151)
+            const R zero = 0;
+            const bool pad = NAC[so] == 0 && KLAS[so] == 0 &&
+                             std::memcmp(&SIDE[so], &zero, sizeof(R)) == 0 &&
+                             std::memcmp(&COSF[so], &zero, sizeof(R)) == 0 &&
+                             std::memcmp(&SINF[so], &zero, sizeof(R)) == 0 &&
+                             std::memcmp(&SLCOS[so], &zero, sizeof(R)) == 0 &&
+                             std::memcmp(&SLSIN[so], &zero, sizeof(R)) == 0;
+            int kind = pad ? K_PAD : kind_of_klas(KLAS[so], prim);
+            meta_n[sn] = (uint8_t)(kind | (prim ? META_PRIMARY_BIT : 0));
+            nbr_n[sn] = (NC >= 0) ? inv[NC] : -1;
+            cos_n[sn] = COSF[so];
+            sin_n[sn] = SINF[so];
+            side_n[sn] = SIDE[so];
+            if (m->need_slcs) {
+                slc_n[sn] = SLCOS[so];
+                sls_n[sn] = SLSIN[so];
+            }
+        }
+    }
+    if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
+        (rc = h2d(m->ZB1, zb1_n)) || (rc = h2d(m->nbr, nbr_n)) || (rc = h2d(m->meta, meta_n)) ||
+        (rc = h2d(m->COSF, cos_n)) || (rc = h2d(m->SINF, sin_n)) || (rc = h2d(m->SIDE, side_n)) ||
+        (rc = h2d(m->SLCOS, slc_n)) || (rc = h2d(m->SLSIN, sls_n)) || (rc = h2d(m->orig, orig_n)))
+        return rc;
+    {
+        const ExtView<R> ev = ext_view<R>(m);
+        if ((rc = m->ext_dev.alloc(sizeof(ExtView<R>)))) return rc;
+        HIP_TRY(hipMemcpy(m->ext_dev.p, &ev, sizeof(ExtView<R>), hipMemcpyHostToDevice));
+    }
+    for (int b = 0; b < 2; b++)
+        if ((rc = m->st[b].alloc((size_t)C * sizeof(State<R>)))) return rc;
+    if ((rc = m->W.alloc((size_t)C * sizeof(R)))) return rc;
+    if (!m->tick.p && (rc = m->tick.alloc(sizeof(int64_t)))) return rc;
+    if (!m->barrier.p && (rc = m->barrier.alloc(sizeof(unsigned)))) return rc;
+    if (!m->perr.p && (rc = m->perr.alloc(sizeof(int)))) return rc;
+    HIP_TRY(hipMemset(m->perr.p, 0, sizeof(int)));
+    m->graphs.clear();
+    m->prepared = true;
+    m->int_valid = false;
+    return HYDRO_OK;
+}
+
+template <class R> int ensure_internal(hydro_mesh *m, hipStream_t s) {
+    if (!m->prepared) {
+        const hydro_tune t = m->tune;
+        int rc = hydro_prepare(m, &t);
+        if (rc) return rc;
+    }
+    if (m->int_valid) return HYDRO_OK;
+    const ExtView<R> ev = ext_view<R>(m);
+    m->cur = 0;
+    HIP_TRY(launch_gather<R>(ev, m->orig.as<int32_t>(), m->st[0].as<State<R>>(), m->CELL, s));
+    HIP_TRY(launch_refresh_meta(m->d.KLAS, m->orig.as<int32_t>(), m->meta.as<uint8_t>(), m->CELL, s));
+    m->int_valid = true;
+    return HYDRO_OK;
+}
+
+template <class R> int ensure_external(hydro_mesh *m, hipStream_t s) {
+    if (m->ext_valid) return HYDRO_OK;
+    const ExtView<R> ev = ext_view<R>(m);
+    HIP_TRY(launch_scatter<R>(ev, m->orig.as<int32_t>(), m->st[m->cur].as<State<R>>(), m->W.as<R>(), m->CELL, s));
+    m->ext_valid = true;
+    return HYDRO_OK;
+}
+
+void advance(hydro_mesh *m) {
+    m->kt++;
+    if (m->kt >= m->spd) {
+        m->kt = 1;
+        m->day++;
+    }
+}
+
+int64_t global_step(const hydro_mesh *m) { return (int64_t)m->day * (m->spd - 1) + (m->kt - 1); }
+
+template <class R, int VAR> int steps_two_kernel(hydro_mesh *m, int n, hipStream_t s) {
+    int rc = ensure_external<R>(m, s);
+    if (rc) return rc;
+    const ExtView<R> ev = ext_view<R>(m);
+    for (int i = 0; i < n; i++) {
+        HIP_TRY((launch_calculate_flux<R, VAR>(ev, m->kt, m->day, m->block_two, s)));
+        HIP_TRY((launch_update_cell<R, VAR>(ev, m->block_two, s)));
+        advance(m);
+    }
+    m->int_valid = false;
+    m->last_launches += 2 * n;
+    return HYDRO_OK;
+}
+
+template <class R, int VAR> int steps_fused(hydro_mesh *m, int n, hipStream_t s) {
+    int rc = ensure_internal<R>(m, s);
+    if (rc) return rc;
+    for (int i = 0; i < n; i++) {
+        const FusedView<R> f = fused_view<R>(m, m->cur);
+        HIP_TRY((launch_fused_step<R, VAR>(f, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
+        m->cur ^= 1;
+        advance(m);
+    }
+    m->ext_valid = false;
+    m->last_launches += n;
+    return HYDRO_OK;
+}
+
+__global__ void k_set_tick(int64_t *tick, int64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *tick = v;
+}
+
+__global__ void k_zero_u32(unsigned *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+}
+
+template <class R, int VAR> int build_graph(hydro_mesh *m, int parity, int chunk, GraphExec &ge) {
+    hipStream_t cs = m->cap_stream;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    int cur = parity;
+    for (int k = 0; k < chunk && err == hipSuccess; k++) {
+        const FusedView<R> f = fused_view<R>(m, cur);
+        err = launch_fused_step_tick<R, VAR>(f, m->tick.as<int64_t>(), k, m->spd, m->tune.block,
+                                             m->tune.xcd_remap, cs);
+        cur ^= 1;
+    }
+    if (err == hipSuccess) err = launch_tick_advance(m->tick.as<int64_t>(), chunk, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &ge.g);
+    if (err != hipSuccess) return fail(HYDRO_ERR_HIP, "graph capture launch failed: %s", hipGetErrorString(err));
+    if (e2 != hipSuccess) return fail(HYDRO_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    HIP_TRY(hipGraphInstantiate(&ge.x, ge.g, nullptr, nullptr, 0));
+    return HYDRO_OK;
+}
+
+template <class R, int VAR> int steps_graph(hydro_mesh *m, int n, hipStream_t s) {
+    int rc = ensure_internal<R>(m, s);
+    if (rc) return rc;
+    int chunk = m->tune.graph_chunk;
+    if (chunk < 2) chunk = 2;
+    if (chunk & 1) chunk++;   // even, so buffer parity is unchanged by one replay
+    if (!m->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking));
+    int done = 0;
+    if (n >= chunk) {
+        GraphKey key{HYDRO_MODE_GRAPH, m->cur, chunk};
+        auto it = m->graphs.find(key);
+        if (it == m->graphs.end()) {
+            auto ge = std::make_unique<GraphExec>();
+            if ((rc = build_graph<R, VAR>(m, m->cur, chunk, *ge))) return rc;
+            it = m->graphs.emplace(key, std::move(ge)).first;
+        }
+        hipLaunchKernelGGL(k_set_tick, dim3(1), dim3(64), 0, s, m->tick.as<int64_t>(), global_step(m));
+        HIP_TRY(hipGetLastError());
+        while (n - done >= chunk) {
+            HIP_TRY(hipGraphLaunch(it->second->x, s));
+            for (int k = 0; k < chunk; k++) advance(m);
+            done += chunk;
+            m->last_launches += chunk;
+        }
+    }
+    if (done < n) return steps_fused<R, VAR>(m, n - done, s);
+    m->ext_valid = false;
+    return HYDRO_OK;
+}
+
+template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream_t s) {
+    int rc = ensure_internal<R>(m, s);
+    if (rc) return rc;
+    if (n == 0) return HYDRO_OK;
+    const int block = 256;
+    int maxg = 0;
+    HIP_TRY((persistent_max_grid<R, VAR>(block, &maxg)));
+    const int ntiles = (m->CELL + block - 1) / block;
+    int want = m->tune.persistent_wgs_per_cu;
+    int grid = maxg;
+    if (want > 0) {
+        int dev = 0, cus = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        grid = std::min(maxg, want * cus);
+    }
+    grid = std::max(1, std::min(grid, ntiles));
+    const FusedView<R> f = fused_view<R>(m, m->cur);
+    PersistentArgs pa;
+    pa.nsteps = n;
+    pa.day0 = m->day;
+    pa.kt0 = m->kt;
+    pa.spd = m->spd;
+    pa.barrier = m->barrier.as<unsigned>();
+    pa.error = m->perr.as<int>();
+    pa.spin_limit = 1u << 24;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY((launch_persistent<R, VAR>(f, pa, block, grid, s)));
+    if (n & 1) m->cur ^= 1;
+    for (int i = 0; i < n; i++) advance(m);
+    m->ext_valid = false;
+    m->last_launches += 1;
+    return HYDRO_OK;
+}
+
+template <class R, int VAR> int step_dispatch(hydro_mesh *m, int n, int mode, hipStream_t s) {
+    switch (mode) {
+    case HYDRO_MODE_TWO_KERNEL: return steps_two_kernel<R, VAR>(m, n, s);
+    case HYDRO_MODE_FUSED: return steps_fused<R, VAR>(m, n, s);
+    case HYDRO_MODE_GRAPH: return steps_graph<R, VAR>(m, n, s);
+    case HYDRO_MODE_PERSISTENT: return steps_persistent<R, VAR>(m, n, s);
+    default: return fail(HYDRO_ERR_UNSUPPORTED, "mode %d not supported", mode);
+    }
+}
+
+hydro_tune default_tune() {
+    hydro_tune t{};
+    t.block = 256;
+    t.reorder = HYDRO_REORDER_RCM;
+    t.xcd_remap = 1;
+    t.graph_chunk = 16;
+    t.persistent_wgs_per_cu = 0;
+    t.materialize_flux = 0;
+    return t;
+}
+
+int check_ptr(const void *p, const char *name, bool required) {
+    if (!p) return required ? fail(HYDRO_ERR_INVALID, "%s is NULL", name) : HYDRO_OK;
+    if (!is_device_ptr(p)) return fail(HYDRO_ERR_INVALID, "%s is not a device pointer", name);
+    return HYDRO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hydro_last_error(void) { return g_err.c_str(); }
+int hydro_abi_version(void) { return HYDRO_ABI_VERSION; }
+
+int hydro_mesh_create(const hydro_mesh_desc *desc, hydro_mesh **out) {
+    if (!desc || !out) return fail(HYDRO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const hydro_mesh_desc &d = *desc;
+    if (d.abi_version != HYDRO_ABI_VERSION)
+        return fail(HYDRO_ERR_INVALID, "abi_version %d, library is %d", d.abi_version, HYDRO_ABI_VERSION);
+    if (d.dtype != HYDRO_F64 && d.dtype != HYDRO_F32) return fail(HYDRO_ERR_INVALID, "dtype %d", d.dtype);
+    if (d.variant != HYDRO_VARIANT_A && d.variant != HYDRO_VARIANT_B)
+        return fail(HYDRO_ERR_INVALID, "variant %d", d.variant);
+    if (d.CELL < 0 || d.CELL > (1 << 28)) return fail(HYDRO_ERR_INVALID, "CELL %d", d.CELL);
+    if (d.NDAYS < 1) return fail(HYDRO_ERR_INVALID, "NDAYS %d", d.NDAYS);
+    if (d.NHQ < 0) return fail(HYDRO_ERR_INVALID, "NHQ %d", d.NHQ);
+    if (!(d.DT > 0) || d.MDT <= 0) return fail(HYDRO_ERR_INVALID, "DT %g MDT %d", d.DT, d.MDT);
+    if (!(d.HM1 > 0) || !(d.HM2 >= d.HM1)) return fail(HYDRO_ERR_INVALID, "HM1 %g HM2 %g", d.HM1, d.HM2);
+    int rc;
+    const bool need = d.CELL > 0;
+#define CHK(ptr, req) \
+    if ((rc = check_ptr(d.ptr, #ptr, (req)))) return rc
+    CHK(H, need); CHK(U, need); CHK(V, need); CHK(Z, need); CHK(W, need);
+    CHK(ZBC, need); CHK(ZB1, need); CHK(AREA, need); CHK(FNC, need);
+    CHK(NAC, need); CHK(KLAS, need); CHK(SIDE, need); CHK(COSF, need); CHK(SINF, need);
+    CHK(SLCOS, need); CHK(SLSIN, need);
+    CHK(FLUX0, need); CHK(FLUX1, need); CHK(FLUX2, need); CHK(FLUX3, need);
+    CHK(BoundaryFeature, false); CHK(NHQ1, false); CHK(ZW, false); CHK(QW, false);
+    CHK(QT, false); CHK(DQT, false); CHK(ZT, false); CHK(DZT, false);
+#undef CHK
+    auto m = std::make_unique<hydro_mesh>();
+    m->d = d;
+    m->CELL = d.CELL;
+    m->rsize = d.dtype == HYDRO_F64 ? 8 : 4;
+    m->spd = (int)(d.MDT / d.DT);
+    m->tune = default_tune();
+    // Host-side validation of the index arrays the kernels dereference.
+    const size_t S4 = 4 * (size_t)d.CELL;
+    std::vector<int32_t> NAC, KLAS;
+    if ((rc = d2h(NAC, d.NAC, S4)) || (rc = d2h(KLAS, d.KLAS, S4))) return rc;
+    for (size_t s = 0; s < S4; s++) {
+        if (NAC[s] < 0 || NAC[s] > d.CELL)
+            return fail(HYDRO_ERR_INVALID, "NAC[%zu] = %d outside [0, CELL=%d]", s, NAC[s], d.CELL);
+        m->has_kind[kind_of_klas(KLAS[s], false)] = true;
+    }
+    if (m->has_kind[K_B10] && (!d.QT || !d.DQT)) return fail(HYDRO_ERR_INVALID, "KLAS 10 slots need QT and DQT");
+    if (m->has_kind[K_B1] && (!d.ZT || !d.DZT)) return fail(HYDRO_ERR_INVALID, "KLAS 1 slots need ZT and DZT");
+    if (m->has_kind[K_B3] && (!d.ZW || !d.QW || !d.NHQ1 || d.NHQ < 2))
+        return fail(HYDRO_ERR_INVALID, "KLAS 3 slots need ZW, QW, NHQ1 and NHQ >= 2");
+    if ((m->has_kind[K_B6] || m->has_kind[K_B7]) && !d.BoundaryFeature)
+        return fail(HYDRO_ERR_INVALID, "KLAS 6/7 slots need BoundaryFeature");
+    if (m->has_kind[K_B3]) {
+        std::vector<int32_t> nhq1;
+        if ((rc = d2h(nhq1, d.NHQ1, d.CELL))) return rc;
+        for (int i = 0; i < d.CELL; i++)
+            if (nhq1[i] < 0 || nhq1[i] > d.NHQ)
+                return fail(HYDRO_ERR_INVALID, "NHQ1[%d] = %d outside [0, NHQ]", i, nhq1[i]);
+    }
+    *out = m.release();
+    return HYDRO_OK;
+}
+
+int hydro_mesh_destroy(hydro_mesh *m) {
+    delete m;
+    return HYDRO_OK;
+}
+
+int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    hydro_tune t = tune ? *tune : default_tune();
+    if (t.block < 64 || t.block > kFusedMaxBlock || (t.block % 64))
+        return fail(HYDRO_ERR_INVALID, "block %d (64..%d, multiple of 64)", t.block, kFusedMaxBlock);
+    if (t.reorder < 0 || t.reorder > 2) return fail(HYDRO_ERR_INVALID, "reorder %d", t.reorder);
+    // the internal state must be written back before the layout is replaced
+    int rc = HYDRO_OK;
+    if (m->prepared && !m->ext_valid)
+        rc = (m->d.dtype == HYDRO_F64) ? ensure_external<double>(m, nullptr) : ensure_external<float>(m, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    m->tune = t;
+    m->prepared = false;
+    return (m->d.dtype == HYDRO_F64) ? prepare_t<double>(m) : prepare_t<float>(m);
+}
+
+int hydro_calculate_flux(hydro_mesh *m, int kt, int jt, hipStream_t stream) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (jt < 0 || jt >= m->d.NDAYS) return fail(HYDRO_ERR_RANGE, "jt %d outside [0, NDAYS=%d)", jt, m->d.NDAYS);
+    int rc;
+    if (m->d.dtype == HYDRO_F64) {
+        if ((rc = ensure_external<double>(m, stream))) return rc;
+        const ExtView<double> ev = ext_view<double>(m);
+        HIP_TRY(m->d.variant ? launch_calculate_flux<double, 1>(ev, kt, jt, m->block_two, stream)
+                             : launch_calculate_flux<double, 0>(ev, kt, jt, m->block_two, stream));
+    } else {
+        if ((rc = ensure_external<float>(m, stream))) return rc;
+        const ExtView<float> ev = ext_view<float>(m);
+        HIP_TRY(m->d.variant ? launch_calculate_flux<float, 1>(ev, kt, jt, m->block_two, stream)
+                             : launch_calculate_flux<float, 0>(ev, kt, jt, m->block_two, stream));
+    }
+    m->int_valid = false;
+    return HYDRO_OK;
+}
+
+int hydro_update_cell(hydro_mesh *m, hipStream_t stream) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    int rc;
+    if (m->d.dtype == HYDRO_F64) {
+        if ((rc = ensure_external<double>(m, stream))) return rc;
+        const ExtView<double> ev = ext_view<double>(m);
+        HIP_TRY(m->d.variant ? launch_update_cell<double, 1>(ev, m->block_two, stream)
+                             : launch_update_cell<double, 0>(ev, m->block_two, stream));
+    } else {
+        if ((rc = ensure_external<float>(m, stream))) return rc;
+        const ExtView<float> ev = ext_view<float>(m);
+        HIP_TRY(m->d.variant ? launch_update_cell<float, 1>(ev, m->block_two, stream)
+                             : launch_update_cell<float, 0>(ev, m->block_two, stream));
+    }
+    m->int_valid = false;
+    return HYDRO_OK;
+}
+
+int hydro_step_n(hydro_mesh *m, int nsteps, int mode, hipStream_t stream) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (nsteps < 0) return fail(HYDRO_ERR_INVALID, "nsteps %d", nsteps);
+    const int per = m->spd - 1;
+    const int64_t total = per > 0 ? (int64_t)m->d.NDAYS * per : 0;
+    const int64_t remaining = total - (per > 0 ? global_step(m) : 0);
+    if (nsteps > remaining)
+        return fail(HYDRO_ERR_RANGE, "%d steps requested, %lld remain before the NDAYS cap", nsteps,
+                    (long long)remaining);
+    m->last_launches = 0;
+    m->last_steps = nsteps;
+    if (m->profiling) {
+        if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
+        if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
+        HIP_TRY(hipEventRecord(m->ev0, stream));
+    }
+    int rc;
+    if (m->d.dtype == HYDRO_F64)
+        rc = m->d.variant ? step_dispatch<double, 1>(m, nsteps, mode, stream)
+                          : step_dispatch<double, 0>(m, nsteps, mode, stream);
+    else
+        rc = m->d.variant ? step_dispatch<float, 1>(m, nsteps, mode, stream)
+                          : step_dispatch<float, 0>(m, nsteps, mode, stream);
+    if (rc) return rc;
+    if (m->profiling) {
+        HIP_TRY(hipEventRecord(m->ev1, stream));
+        HIP_TRY(hipEventSynchronize(m->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        m->last_ms = ms;
+    }
+    return HYDRO_OK;
+}
+
+int hydro_set_time(hydro_mesh *m, int day, int kt) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (day < 0 || day > m->d.NDAYS || kt < 1 || (kt >= m->spd && m->spd > 1))
+        return fail(HYDRO_ERR_RANGE, "time (day %d, kt %d) outside the step grid", day, kt);
+    m->day = day;
+    m->kt = kt;
+    return HYDRO_OK;
+}
+
+int hydro_get_time(const hydro_mesh *m, int *day, int *kt, int *steps_per_day) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (day) *day = m->day;
+    if (kt) *kt = m->kt;
+    if (steps_per_day) *steps_per_day = m->spd;
+    return HYDRO_OK;
+}
+
+int hydro_sync(hydro_mesh *m, hipStream_t stream) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    return (m->d.dtype == HYDRO_F64) ? ensure_external<double>(m, stream) : ensure_external<float>(m, stream);
+}
+
+int hydro_reload(hydro_mesh *m, hipStream_t stream) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    m->ext_valid = true;
+    m->int_valid = false;
+    if (!m->prepared) return HYDRO_OK;
+    return (m->d.dtype == HYDRO_F64) ? ensure_internal<double>(m, stream) : ensure_internal<float>(m, stream);
+}
+
+int hydro_dump(hydro_mesh *m, const char *path, hipStream_t stream) {
+    if (!m || !path) return fail(HYDRO_ERR_INVALID, "null argument");
+    int rc = hydro_sync(m, stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    const int C = m->CELL;
+    const size_t rs = (size_t)m->rsize, S4 = 4 * (size_t)C;
+    std::vector<char> buf(rs * S4);
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail(HYDRO_ERR_INVALID, "cannot open %s", path);
+    fwrite(&C, sizeof(int32_t), 1, fp);
+    const void *cellv[4] = {m->d.H, m->d.U, m->d.V, m->d.Z};
+    for (const void *p : cellv) {
+        if (hipMemcpy(buf.data(), p, rs * C, hipMemcpyDeviceToHost) != hipSuccess) {
+            fclose(fp);
+            return fail(HYDRO_ERR_HIP, "dump copy failed");
+        }
+        fwrite(buf.data(), rs, C, fp);
+    }
+    const void *slotv[7] = {m->d.SLCOS, m->d.SLSIN, m->d.SIDE, m->d.FLUX0, m->d.FLUX1, m->d.FLUX2, m->d.FLUX3};
+    for (const void *p : slotv) {
+        if (hipMemcpy(buf.data(), p, rs * S4, hipMemcpyDeviceToHost) != hipSuccess) {
+            fclose(fp);
+            return fail(HYDRO_ERR_HIP, "dump copy failed");
+        }
+        fwrite(buf.data(), rs, S4, fp);
+    }
+    fclose(fp);
+    return HYDRO_OK;
+}
+
+int hydro_set_profiling(hydro_mesh *m, int enable) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    m->profiling = enable != 0;
+    return HYDRO_OK;
+}
+
+int hydro_last_timing(const hydro_mesh *m, double *total_ms, int *launches, int *steps) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (total_ms) *total_ms = m->last_ms;
+    if (launches) *launches = m->last_launches;
+    if (steps) *steps = m->last_steps;
+    return HYDRO_OK;
+}
+
+int hydro_precalculate(int dtype, int CELL, int NOD, const int32_t *NAP, void *XP, void *YP,
+                       const void *ZBC, void *Z, const void *FNC0, double HM1, void *SIDE, void *SINF,
+                       void *COSF, void *SLCOS, void *SLSIN, void *AREA, void *ZB1, void *H, void *FNC) {
+    if (CELL < 0 || NOD < 0) return fail(HYDRO_ERR_INVALID, "CELL %d NOD %d", CELL, NOD);
+    for (int i = 0; i < 4 * CELL; i++)
+        if (NAP[i] < 0 || NAP[i] > NOD) return fail(HYDRO_ERR_INVALID, "NAP[%d] = %d outside [0, NOD]", i, NAP[i]);
+    if (dtype == HYDRO_F64)
+        precalculate<double>(CELL, NOD, NAP, (double *)XP, (double *)YP, (const double *)ZBC, (double *)Z,
+                             (const double *)FNC0, HM1, (double *)SIDE, (double *)SINF, (double *)COSF,
+                             (double *)SLCOS, (double *)SLSIN, (double *)AREA, (double *)ZB1, (double *)H,
+                             (double *)FNC);
+    else if (dtype == HYDRO_F32)
+        precalculate<float>(CELL, NOD, NAP, (float *)XP, (float *)YP, (const float *)ZBC, (float *)Z,
+                            (const float *)FNC0, (float)HM1, (float *)SIDE, (float *)SINF, (float *)COSF,
+                            (float *)SLCOS, (float *)SLSIN, (float *)AREA, (float *)ZB1, (float *)H,
+                            (float *)FNC);
+    else
+        return fail(HYDRO_ERR_INVALID, "dtype %d", dtype);
+    return HYDRO_OK;
+}
+
+int hydro_rcm_order(int CELL, const int32_t *NAC, int32_t *perm) {
+    if (CELL < 0 || (!NAC && CELL) || (!perm && CELL)) return fail(HYDRO_ERR_INVALID, "bad arguments");
+    for (int i = 0; i < 4 * CELL; i++)
+        if (NAC[i] < 0 || NAC[i] > CELL) return fail(HYDRO_ERR_INVALID, "NAC[%d] = %d", i, NAC[i]);
+    std::vector<int32_t> p = rcm_order(CELL, NAC);
+    std::memcpy(perm, p.data(), sizeof(int32_t) * CELL);
+    return HYDRO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// kernels.hpp -- views and launch helpers shared by kernels.hip and api.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "physics.hpp"
+
+namespace hydro {
+
+// The caller's arrays in the reference layout (hydro_mesh_desc, typed).
+template <class R> struct ExtView {
+    int CELL, NDAYS, NHQ;
+    Scalars<R> S;
+    R *H, *U, *V, *Z, *W;
+    const R *ZBC, *ZB1, *AREA, *FNC, *BF;
+    const int32_t *NHQ1;
+    const R *ZW, *QW, *QT, *DQT, *ZT, *DZT;
+    const int32_t *NAC;
+    int32_t *KLAS;
+    const R *SIDE, *COSF, *SINF, *SLCOS, *SLSIN;
+    R *F0, *F1, *F2, *F3;
+};
+
+template <class R> struct alignas(4 * sizeof(R)) State {
+    R H, U, V, Z;
+};
+
+// The reordered layout of the fused paths.  Cell p of the new order is
+// original cell orig[p].  Slot arrays are slot-major: element [j*C + p].
+template <class R> struct FusedView {
+    int C;
+    int materialize;           // also write the caller's FLUX0..3 every step
+    Scalars<R> S;
+    // the caller's arrays, read only by boundary slots, KLAS 7 rewrites and
+    // materialised fluxes; kept behind a device pointer so the common path
+    // does not hold them in scalar registers
+    const ExtView<R> *ext;
+    // ping-pong state records {H, U, V, Z}; `in` is read, `out` written,
+    // swapped every step.  One record per cell keeps a neighbour gather to
+    // one 32-byte (fp64) / 16-byte (fp32) load instead of four.
+    const State<R> *Sin;
+    State<R> *Sout;
+    R *W;
+    const R *ZBC, *AREA, *FNC;
+    const R *ZB1;              // null: ZB1 == ZBC + HM1 bitwise for every cell
+    const int32_t *nbr;        // [4][C] new-order neighbour, -1 none
+    uint8_t *meta;             // [4][C] kind | primary bit (KLAS 7 slots are rewritten)
+    const R *COSF, *SINF, *SIDE;    // [4][C]
+    const R *SLCOS, *SLSIN;    // [4][C] or null when SIDE*COSF / SIDE*SINF reproduce them
+    const int32_t *orig;       // [C]
+};
+
+constexpr int kMaxBlock = 1024;
+constexpr int kFusedMaxBlock = 256;   // fp64 fused step needs > 128 VGPRs
+
+// Launch helpers implemented in kernels.hip (explicit instantiations there).
+template <class R, int VAR>
+hipError_t launch_calculate_flux(const ExtView<R> &m, int kt, int jt, int block, hipStream_t s);
+template <class R, int VAR>
+hipError_t launch_update_cell(const ExtView<R> &m, int block, hipStream_t s);
+template <class R, int VAR>
+hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int xcd_remap, hipStream_t s);
+template <class R, int VAR>
+hipError_t launch_fused_step_tick(const FusedView<R> &f, const int64_t *tick, int offset, int spd, int block,
+                                  int xcd_remap, hipStream_t s);
+hipError_t launch_tick_advance(int64_t *tick, int by, hipStream_t s);
+
+struct PersistentArgs {
+    int nsteps;
+    int day0, kt0, spd;
+    unsigned *barrier;   // zeroed before launch
+    int *error;          // set to 1 if a barrier wait exceeded its spin limit
+    unsigned spin_limit;
+};
+// f reads f.Sin on even steps and f.Sout on odd steps.
+template <class R, int VAR>
+hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, int block, int grid, hipStream_t s);
+template <class R, int VAR>
+hipError_t persistent_max_grid(int block, int *grid);
+
+// external H U V Z -> internal records (new order)
+template <class R>
+hipError_t launch_gather(const ExtView<R> &m, const int32_t *orig, State<R> *st, int C, hipStream_t s);
+// internal records and W -> external H U V Z W
+template <class R>
+hipError_t launch_scatter(const ExtView<R> &m, const int32_t *orig, const State<R> *st, const R *W,
+                          int C, hipStream_t s);
+hipError_t launch_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uint8_t *meta, int C,
+                               hipStream_t s);
+
+}  // namespace hydro

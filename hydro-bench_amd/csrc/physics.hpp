@@ -1,0 +1,541 @@
+// physics.hpp -- per-slot flux and per-cell update of the hydro-cal timestep,
+// device side, templated on Real (double / float) and on the variant
+// (0 = A: cuda_native_impl/hydro-cal-src/src/functors.cu,
+//  1 = B: cuda_native_impl/baselines/native_cuda_v0/fp64/hydro-cal-src/src/functors.cu).
+//
+// Arithmetic is written so that every rounding happens where the reference's
+// does: Real-typed locals, double literals (so the float build promotes the
+// same sub-expressions to double that `using Real = float` does in
+// cuda_native_impl/fp32_src), sqrt_r = sqrt / sqrtf, pow in double.  The
+// library is compiled with -ffp-contract=off so no FMA changes a rounding.
+//
+// Structure differs from the reference on purpose (MI355X-first):
+//  * the flux of a slot is returned in registers; the caller decides whether
+//    it is stored (two-kernel path) or consumed by the fused cell update;
+//  * the primary-side and mirror-side Osher solves (F:151-200) are one
+//    Riemann solve on (left, right) states chosen with selects, so a wave
+//    with both kinds of slot runs the solver once, not twice;
+//  * the 16-case Osher table (F:612-711) is evaluated as five predicated
+//    terms in the table's fixed order (left state, left sonic point,
+//    intermediate state, right sonic point, right state), each computed at
+//    most once per lane and skipped by the whole wave when no lane needs it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hydro {
+
+// Slot kinds, one per slot of the reordered layout (low 4 bits of the slot
+// meta byte).  The first two are interior slots that reach the Osher branch
+// as the primary side (KLAS 0 and original cell id < original neighbour id,
+// F:151) or as the mirror side (any other non-boundary KLAS, F:165).
+enum SlotKind : int {
+    K_PRIMARY = 0,
+    K_MIRROR = 1,
+    K_B10 = 2,   // discharge            (F:204-232)
+    K_B1 = 3,    // water level          (F:371-392)
+    K_B3 = 4,    // rating curve         (F:320-369)
+    K_B4 = 5,    // wall                 (F:544-550)
+    K_B5 = 6,    // open                 (F:551-557)
+    K_B6 = 7,    // weir                 (F:258-318)
+    K_B7 = 8,    // breach               (F:234-256)
+    K_BUNK = 9,  // boundary code without a handler: only FLUX2 is written
+    K_PAD = 15   // triangle padding slot (NAC 0, SIDE 0, KLAS 0, COSF SINF 0)
+};
+constexpr int META_KIND_MASK = 0x0f;
+constexpr int META_PRIMARY_BIT = 0x10;  // kind to take if a KLAS 7 slot becomes 0
+
+// KLAS -> kind (F:109 decides boundary vs interior; F:151 primary vs mirror).
+__host__ __device__ inline int kind_of_klas(int klas, bool primary_side) {
+    if ((klas >= 1 && klas <= 8) || klas >= 10) {
+        switch (klas) {
+        case 10: return K_B10;
+        case 1: return K_B1;
+        case 3: return K_B3;
+        case 4: return K_B4;
+        case 5: return K_B5;
+        case 6: return K_B6;
+        case 7: return K_B7;
+        default: return K_BUNK;
+        }
+    }
+    return (klas == 0 && primary_side) ? K_PRIMARY : K_MIRROR;
+}
+
+template <class R> struct Consts {
+    static constexpr R C0 = (R)1.33;
+    static constexpr R C1 = (R)1.7;
+    static constexpr R VMIN = (R)0.001;
+    static constexpr R QLUA = (R)0.0;
+    static constexpr R BRDTH = (R)100.0;
+};
+
+template <class R> __host__ __device__ __forceinline__ R sqrt_r(double x) { return sqrtf((float)x); }
+template <> __host__ __device__ __forceinline__ double sqrt_r<double>(double x) { return sqrt(x); }
+__host__ __device__ __forceinline__ double pow_d(double x, double y) { return pow(x, y); }
+
+template <class R> struct Flux { R f0, f1, f2, f3; };
+
+// Mesh-wide scalars as the kernels use them.
+template <class R> struct Scalars {
+    R DT, HM1, HM2, JL;
+};
+
+// Everything the boundary handlers may read besides the cell states.
+template <class R> struct BcCtx {
+    const R *QT, *DQT, *ZT, *DZT, *ZW, *QW, *BF;
+    const int32_t *NHQ1;
+    int CELL, NHQ;
+    int cell;   // ORIGINAL cell id (rows of the dense boundary arrays)
+    int kt, jt;
+    R SIDE, COSF, SINF;
+};
+
+// ---------------------------------------------------------------------------
+// Osher approximate Riemann flux (F:570-712 with QS F:445-503, QF F:505-511).
+template <class R>
+__host__ __device__ __forceinline__ void qf_acc(R acc[4], R h, R u, R v, bool neg) {
+    R f0 = h * u;
+    R f1 = f0 * u;
+    R f2 = f0 * v;
+    R f3 = 4.905 * h * h;
+    if (neg) { f0 = -f0; f1 = -f1; f2 = -f2; f3 = -f3; }
+    acc[0] = acc[0] + f0;
+    acc[1] = acc[1] + f1;
+    acc[2] = acc[2] + f2;
+    acc[3] = acc[3] + f3;
+}
+
+template <class R>
+__host__ __device__ __forceinline__ void osher(R H_pre, R QL0, R QL1, R QL2, R QR0, R QR1, R QR2,
+                                               R FIL, R out[4]) {
+    const R CR = sqrt_r<R>(9.81 * QR0);
+    const R FIR = QR1 - 2 * CR;
+    const R UA = (FIL + FIR) / 2;
+    const R CA = fabs((FIL - FIR) / 4);
+    const R CL = sqrt_r<R>(9.81 * H_pre);
+    // K2 from the intermediate state, K1 from the end states (F:586-610;
+    // the cases the reference leaves undefined take 4, as in the Taichi port).
+    int K2 = (CA < UA) ? 1 : (UA >= 0.0 && UA < CA) ? 2 : (UA >= -CA && UA < 0.0) ? 3 : 4;
+    const bool rsub = (QR1 >= -CR);
+    int K1 = (QL1 < CL && rsub) ? 1 : (QL1 >= CL && rsub) ? 2 : (QL1 < CL && QR1 < -CR) ? 3 : 4;
+    const bool lsup = (K1 == 2 || K1 == 4);   // flux starts from the left state
+    const bool rneg = (K1 == 3 || K1 == 4);   // flux ends at the right state
+    out[0] = 0; out[1] = 0; out[2] = 0; out[3] = 0;
+    if (lsup) qf_acc<R>(out, QL0, QL1, QL2, false);
+    if (lsup ? (K2 != 1) : (K2 == 1)) {          // sonic point of the left curve
+        R US = FIL / 3;
+        R HS = US * US / 9.81;
+        qf_acc<R>(out, HS, US, QL2, lsup);
+    }
+    R FIRm = FIR;
+    if (K2 == 2 || K2 == 3) {                    // intermediate state
+        const bool left = (K2 == 2);
+        R X = (left ? FIL : FIR) - UA;
+        if (!left) FIRm = X;                     // QS<5> updates FIR in place
+        R HA = X * X / 39.24;
+        qf_acc<R>(out, HA, UA, left ? QL2 : QR2, false);
+    }
+    if (rneg ? (K2 != 4) : (K2 == 4)) {          // sonic point of the right curve
+        R US = FIRm / 3;
+        R HS = US * US / 9.81;
+        qf_acc<R>(out, HS, US, QR2, rneg);
+    }
+    if (rneg) qf_acc<R>(out, QR0, QR1, QR2, false);
+}
+
+// ---------------------------------------------------------------------------
+// QD (F:394-421) and LAQP (F:423-443).
+template <class R> __host__ __device__ inline R qd(R ZL, R ZR, R ZB) {
+    const R CM = 0.384, SIGMA = 0.667, FI = 4.43;
+    R ZU = fmax(ZL, ZR), ZD = fmin(ZL, ZR);
+    R H0 = ZU - ZB, HS = ZD - ZB;
+    R DELTA = HS / H0;
+    if (DELTA <= SIGMA) return copysign(CM * pow_d(H0, 1.5), (double)(ZL - ZR));
+    R DH = ZU - ZD;
+    if (DH > 0.09) return copysign(FI * HS * sqrt_r<R>(DH), ZL - ZR);
+    return copysign(FI * HS * 0.3 * DH / 0.1, (double)(ZL - ZR));
+}
+
+template <class R> __host__ __device__ inline R laqp(R X, const R *A, const R *B, int MS) {
+    if (MS <= 0) return 0;
+    if (X < A[0]) return B[0];
+    if (X > A[MS - 1]) return B[MS - 1];
+    for (int i = 0; i < MS - 1; i++)
+        if (X >= A[i] && X <= A[i + 1]) return B[i] + (B[i + 1] - B[i]) / (A[i + 1] - A[i]) * (X - A[i]);
+    return 0;
+}
+
+template <class R> __host__ __device__ inline R urb_iterate(R QLn, R H_pre, R HB1, double tol) {
+    R FIAL = QLn + 6.264 * sqrt_r<R>(H_pre);
+    R UR0 = QLn;
+    R URB = UR0;
+    for (int it = 1; it <= 30; it++) {
+        R FIAR = URB - 6.264 * sqrt_r<R>(HB1);
+        URB = (FIAL + FIAR) * (FIAL - FIAR) * (FIAL - FIAR) / HB1 / 313.92;
+        if (fabs(URB - UR0) <= tol) break;
+        UR0 = URB;
+    }
+    return URB;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary slot (BOUNDA F:513-568).  `prev` supplies FLUX0/1/3 for codes
+// that do not write them.  Sets *klas7_fired when a KLAS 7 slot turns into
+// an interior slot (F:242).
+template <class R, int VAR>
+__host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
+                                                 R H1, R U1, R V1, R Z1, R ZBp, R ZC, R UC, R VC,
+                                                 R QL1, R QL2, R FIL, Flux<R> prev,
+                                                 bool *klas7_fired) {
+    using K = Consts<R>;
+    Flux<R> f;
+    const R CL = sqrt_r<R>(9.81 * H1);
+    const bool super = (VAR == 1) ? (QL1 > CL) : (QL1 > CL && H1 < S.HM2);
+    if (super) {
+        f.f0 = H1 * QL1;
+        f.f1 = f.f0 * QL1;
+        f.f2 = f.f0 * QL2;
+        f.f3 = 4.905 * H1 * H1;
+        return f;
+    }
+    f.f2 = 0;
+    if (QL1 > 0) f.f2 = H1 * QL1 * QL2;
+    f.f0 = prev.f0;
+    f.f1 = prev.f1;
+    f.f3 = prev.f3;
+    const size_t row = (size_t)bc.jt * (size_t)bc.CELL + (size_t)bc.cell;
+    switch (kind) {
+    case K_B10: {
+        R F0 = -(bc.QT[row] + bc.DQT[row] * bc.kt);
+        F0 = F0 / bc.SIDE;
+        R QB2 = F0 * F0;
+        R HB0 = H1;
+        R HB = 0;
+        for (int k = 1; k <= 20; k++) {
+            R Wt = FIL - F0 / HB0;
+            HB = Wt * Wt / 39.24;
+            if (fabs(HB0 - HB) <= 0.005) break;
+            HB0 = HB0 * 0.5 + HB * 0.5;
+        }
+        f.f0 = F0;
+        f.f1 = (HB <= 1) ? (R)0 : QB2 / HB;
+        f.f2 = 0;
+        f.f3 = 4.905 * HB * HB;
+    } break;
+    case K_B1: {
+        R HB1 = bc.ZT[row] + bc.DZT[row] * (VAR == 1 ? bc.kt : bc.jt) - ZBp;
+        HB1 = fmax(HB1, S.HM2);
+        R URB = urb_iterate<R>(QL1, H1, HB1, 0.0001);
+        f.f0 = HB1 * URB;
+        f.f1 = f.f0 * URB;
+        f.f3 = 4.905 * HB1 * HB1;
+    } break;
+    case K_B3: {
+        R QZH3 = (U1 * bc.COSF + V1 * bc.SINF) * H1 * bc.SIDE;
+        QZH3 = fmax(QZH3, (R)0.0);
+        const R *QWr = bc.QW + (size_t)bc.cell * (size_t)bc.NHQ;
+        const R *ZWr = bc.ZW + (size_t)bc.cell * (size_t)bc.NHQ;
+        R ZQH1 = laqp<R>(QZH3, QWr, ZWr, bc.NHQ1[bc.cell]);
+        R ZQH11 = ZQH1 - Z1;
+        R HB1 = ZQH1 - ZBp;
+        HB1 = fmax(HB1, S.HM2);
+        R q = fmax(QL1, (R)0.0);
+        if (QZH3 <= QWr[1] || HB1 <= S.HM2 || ZQH11 >= 0.1) {
+            f.f0 = H1 * q;
+            f.f1 = f.f0 * q;
+            f.f3 = 4.905 * H1 * H1;
+        } else {
+            R URB = urb_iterate<R>(q, H1, HB1, 0.001);
+            f.f0 = HB1 * URB;
+            f.f1 = f.f0 * URB;
+            f.f3 = 4.905 * HB1 * HB1;
+        }
+    } break;
+    case K_B4:
+        f.f0 = 0; f.f1 = 0; f.f2 = 0;
+        f.f3 = 4.905 * H1 * H1;
+        break;
+    case K_B5: {
+        R q = fmax(QL1, (R)0.0);
+        f.f0 = H1 * q;
+        f.f1 = f.f0 * q;
+        f.f3 = 4.905 * H1 * H1 * (1.0 - S.JL) * (1.0 - S.JL);
+    } break;
+    case K_B6: {
+        const R TOP = bc.BF[bc.cell];
+        const R cs = bc.COSF, sn = bc.SINF;
+        if (Z1 <= TOP && ZC <= TOP) {
+            f.f0 = 0; f.f1 = 0; f.f2 = 0;
+            f.f3 = 4.905 * H1 * H1;
+        } else if (Z1 > TOP && ZC <= TOP) {
+            f.f0 = K::C0 * pow_d(Z1 - TOP, 1.5);
+            f.f1 = f.f0 * QL1;
+            f.f2 = f.f0 * QL2;
+            f.f3 = 4.905 * pow_d(TOP - ZBp, 2.0);
+        } else if (Z1 <= TOP && ZC > TOP) {
+            f.f0 = -K::C0 * pow_d(ZC - TOP, 1.5);
+            f.f1 = f.f0 * fmin(UC * cs + VC * sn, (R)0.0);
+            f.f2 = f.f0 * (VC * cs - UC * sn);
+            f.f3 = 4.905 * pow_d(Z1 - ZBp, 2.0);
+        } else {
+            R DZ = fabs(Z1 - ZC);
+            if (Z1 <= ZC) {
+                R HD = Z1 - TOP;
+                R UN = fmin(UC * cs + VC * sn, (R)0.0);
+                R VT = VC * cs - UC * sn;
+                R SH = HD + DZ;
+                R CE = fmin(1.0, 1.05 * pow_d(DZ / SH, 0.33333));
+                if (Z1 < ZC && UN > 0.0) UN = 0.0;
+                f.f0 = copysign(CE * K::C1 * pow_d(SH, 1.5), (double)(Z1 - ZC));
+                f.f1 = f.f0 * fabs(UN);
+                f.f2 = f.f0 * VT;
+            } else {
+                R HD = ZC - TOP;
+                R UN = fmax(QL1, (R)0.0);
+                R VT = QL2;
+                R SH = HD + DZ;
+                R CE = fmin(1.0, 1.05 * pow_d(DZ / SH, 0.33333));
+                f.f0 = copysign(CE * K::C1 * pow_d(SH, 1.5), (double)(Z1 - ZC));
+                f.f1 = (Z1 - ZC) * fabs(UN) * UN;
+                f.f2 = (Z1 - ZC) * fabs(UN) * VT;
+            }
+            f.f3 = 4.905 * pow_d(TOP - ZBp, 2.0);
+        }
+    } break;
+    case K_B7: {
+        const R TOP = bc.BF[bc.cell];
+        const R HB = 0;   // the reference reads an uninitialised HB here
+        if (Z1 > TOP || ZC > TOP) {
+            *klas7_fired = true;
+            R CQ = qd<R>(Z1, ZC, TOP);
+            R CB = K::BRDTH / bc.SIDE;
+            f.f0 = CQ * CB;
+            f.f1 = CB * copysign(CQ * CQ / HB, CQ);
+            f.f3 = 4.905 * HB * HB;
+        } else {
+            f.f0 = 0; f.f1 = 0; f.f2 = 0;
+            f.f3 = 4.905 * H1 * H1;
+        }
+    } break;
+    default:  // K_BUNK: FLUX0/1/3 keep their previous values
+        break;
+    }
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// Flux of one non-padding slot (CalculateFluxKernel body F:47-202).
+//   own:  H1 U1 V1 Z1, ZB1 (own ZB1 array value), ZBp (own bed)
+//   nbr:  Hn Un Vn Zn ZBn, valid when has_nbr (NAC != 0)
+template <class R, int VAR>
+__host__ __device__ inline Flux<R> slot_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
+                                             R H1, R U1, R V1, R Z1, R ZB1, R ZBp,
+                                             bool has_nbr, R Hn, R Un, R Vn, R Zn, R ZBn,
+                                             R COSJ, R SINJ, Flux<R> prev, bool *klas7_fired) {
+    using K = Consts<R>;
+    const R QL1 = U1 * COSJ + V1 * SINJ;
+    const R QL2 = V1 * COSJ - U1 * SINJ;
+    const R ZI = fmax(Z1, ZB1);
+    R HC = 0, BC = 0, ZC = 0, UC = 0, VC = 0;
+    if (has_nbr) {
+        HC = fmax(Hn, S.HM1);
+        BC = ZBn;
+        ZC = fmax(BC, Zn);
+        UC = Un;
+        VC = Vn;
+    }
+    Flux<R> f;
+    if (kind >= K_B10) {
+        const R CL = sqrt_r<R>(9.81 * H1);
+        const R FIL = QL1 + 2 * CL;
+        return boundary_flux<R, VAR>(kind, S, bc, H1, U1, V1, Z1, ZBp, ZC, UC, VC, QL1, QL2, FIL,
+                                     prev, klas7_fired);
+    }
+    if (H1 <= S.HM1 && HC <= S.HM1) {
+        f.f0 = 0; f.f1 = 0; f.f2 = 0; f.f3 = 0;
+        return f;
+    }
+    if (ZI <= BC) {
+        f.f0 = -K::C1 * pow_d(HC, 1.5);
+        f.f1 = H1 * QL1 * fabs(QL1);
+        f.f2 = 0;
+        f.f3 = 4.905 * H1 * H1;
+        return f;
+    }
+    if (ZC <= ZBp) {
+        f.f0 = K::C1 * pow_d(H1, 1.5);
+        f.f1 = H1 * fabs(QL1) * QL1;
+        f.f2 = H1 * fabs(QL1) * QL2;
+        f.f3 = 0;
+        return f;
+    }
+    if (H1 <= S.HM2) {
+        if (ZC > ZI) {
+            R DH = fmax(ZC - ZBp, S.HM1);
+            R UN = -K::C1 * sqrt_r<R>(DH);
+            f.f0 = DH * UN;
+            f.f1 = f.f0 * UN;
+            f.f2 = f.f0 * (VC * COSJ - UC * SINJ);
+        } else {
+            f.f0 = K::C1 * pow_d(H1, 1.5);
+            f.f1 = 0;
+            f.f2 = 0;
+        }
+        f.f3 = 4.905 * H1 * H1;
+        return f;
+    }
+    if (HC <= S.HM2) {
+        if (ZI > ZC) {
+            R DH = fmax(ZI - BC, S.HM1);
+            R UN = K::C1 * sqrt_r<R>(DH);
+            R HC1 = ZC - ZBp;
+            f.f0 = DH * UN;
+            f.f1 = f.f0 * UN;
+            f.f2 = f.f0 * QL2;
+            f.f3 = 4.905 * HC1 * HC1;
+        } else {
+            f.f0 = -K::C1 * pow_d(HC, 1.5);
+            f.f1 = H1 * QL1 * QL1;
+            f.f2 = 0;
+            f.f3 = 4.905 * H1 * H1;
+        }
+        return f;
+    }
+    // Both sides wet: one Riemann solve on (left a, right b).  Primary side:
+    // a = own, b = neighbour, normal (COSJ, SINJ).  Mirror side: a = neighbour,
+    // b = own, normal negated -- the neighbour's own primary problem.
+    const bool prim = (kind == K_PRIMARY);
+    const R c = prim ? COSJ : -COSJ;
+    const R s = prim ? SINJ : -SINJ;
+    const R Ha = prim ? H1 : Hn, Ua = prim ? U1 : Un, Va = prim ? V1 : Vn, ZBa = prim ? ZBp : ZBn;
+    const R Hb = prim ? Hn : H1, Ub = prim ? Un : U1, Vb = prim ? Vn : V1;
+    const R ZBb = prim ? ZBn : ZBp, Zb = prim ? Zn : Z1;
+    const R qa1 = Ua * c + Va * s;
+    const R qa2 = Va * c - Ua * s;
+    const R CLa = sqrt_r<R>(9.81 * Ha);
+    const R FILa = qa1 + 2 * CLa;
+    const R ZCb = fmax(ZBb, Zb);
+    const R HCb = fmax(Hb, S.HM1);
+    const R QR0 = fmax(ZCb - ZBa, S.HM1);
+    const R UR = Ub * c + Vb * s;
+    const R ratio = fmin(HCb / QR0, (R)1.5);
+    R QR1 = UR * ratio;
+    if (HCb <= S.HM2 || QR0 <= S.HM2) QR1 = copysign(K::VMIN, UR);
+    const R QR2 = Vb * c - Ub * s;
+    R FO[4];
+    osher<R>(Ha, Ha, qa1, qa2, QR0, QR1, QR2, FILa, FO);
+    f.f1 = FO[1] + (1 - fmin(HCb / QR0, (R)1.5)) * HCb * UR * UR / 2;
+    f.f2 = FO[2];
+    if (prim) {
+        f.f0 = FO[0];
+        f.f3 = FO[3];
+    } else {
+        f.f0 = -FO[0];
+        R ZA = sqrt_r<R>(FO[3] / 4.905) + ZBa;
+        R HC3 = 0;
+        if (ZA > ZBp) HC3 = ZA - ZBp;
+        f.f3 = 4.905 * HC3 * HC3;
+    }
+    return f;
+}
+
+// Sign-exact stand-in for the flux of a triangle's padding slot (NAC 0,
+// SIDE COSF SINF 0, KLAS 0).  The reference computes a flux there and adds
+// SIDE*FLUX = +-0 into the cell sums; only the signs of those zeros can
+// matter (they decide the sign of an exactly-zero WU, which copysign later
+// reads).  This evaluates the same branch ladder with every positive
+// magnitude (pow, sqrt) replaced by a positive stand-in, so SIDE*f gives the
+// reference's zero, bit for bit, without the transcendental work.
+template <class R>
+__host__ __device__ __forceinline__ Flux<R> padding_flux_signs(const Scalars<R> &S, R H1, R U1, R V1,
+                                                               R Z1, R ZB1, R ZBp) {
+    const R QL1 = U1 * (R)0 + V1 * (R)0;
+    const R QL2 = V1 * (R)0 - U1 * (R)0;
+    const R ZI = fmax(Z1, ZB1);
+    Flux<R> f;
+    if (H1 <= S.HM1) {                      // both dry (HC = 0)
+        f.f0 = 0; f.f1 = 0; f.f2 = 0; f.f3 = 0;
+    } else if (ZI <= (R)0) {                // BC = 0
+        f.f0 = -(R)0;                       // -C1 * pow(0, 1.5)
+        f.f1 = H1 * QL1 * fabs(QL1);
+        f.f2 = 0;
+        f.f3 = 1;
+    } else if ((R)0 <= ZBp) {               // ZC = 0
+        f.f0 = 1;
+        f.f1 = H1 * fabs(QL1) * QL1;
+        f.f2 = H1 * fabs(QL1) * QL2;
+        f.f3 = 0;
+    } else if (H1 <= S.HM2) {               // ZC = 0 < ZI here: no inflow form
+        f.f0 = 1; f.f1 = 0; f.f2 = 0; f.f3 = 1;
+    } else {                                // HC = 0 <= HM2 and ZI > ZC = 0
+        f.f0 = 1; f.f1 = 1;
+        f.f2 = QL2;                         // F0 * QL2 with F0 > 0
+        f.f3 = 1;
+    }
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// UpdateCellKernel arithmetic after the flux sums (F:744-783; variant B's
+// per-cell CFL Fv0:751-768 and 5 m/s cap Fv0:791-793).
+template <class R> struct CellOut { R H, U, V, Z, W; };
+
+template <class R, int VAR>
+__host__ __device__ __forceinline__ CellOut<R> update_math(const Scalars<R> &S, R H1, R U1, R V1,
+                                                           R ZBp, R AREA, R FNC, R WH, R WU, R WV,
+                                                           R s0, R s1, R s2, R s3) {
+    using K = Consts<R>;
+    R DTA;
+    if (VAR == 1) {
+        R SIDEX;
+        if (s3 > (R)0.0) {
+            SIDEX = fmin((R)0.5 * (s0 + s2), (R)0.5 * (s1 + s3));
+        } else {
+            R SIDES = (R)0.5 * (s0 + s1 + s2);
+            SIDEX = sqrt_r<R>((SIDES - s0) * (SIDES - s1) * (SIDES - s2) / SIDES);
+        }
+        R HSIDE = fmax(H1, S.HM1);
+        R DT2 = SIDEX / (U1 + sqrt_r<R>((R)9.81 * HSIDE));
+        DT2 = fmin((R)S.DT, DT2);
+        DT2 = fmax(DT2, (R)S.DT / (R)10.0);
+        DTA = DT2 / AREA;
+    } else {
+        DTA = 1.0 * S.DT / (1.0 * AREA);
+    }
+    const R WDTA = 1.00 * DTA;
+    CellOut<R> o;
+    o.H = fmax(H1 - WDTA * WH + K::QLUA, S.HM1);
+    o.Z = o.H + ZBp;
+    if (o.H <= S.HM1) {
+        o.U = 0.0;
+        o.V = 0.0;
+    } else if (o.H <= S.HM2) {
+        o.U = copysign(fmin(K::VMIN, fabs(U1)), U1);
+        o.V = copysign(fmin(K::VMIN, fabs(V1)), V1);
+    } else {
+        const R QX1 = H1 * U1, QY1 = H1 * V1;
+        const R DTAU = WDTA * WU, DTAV = WDTA * WV;
+        const R WSF = FNC * sqrt_r<R>(U1 * U1 + V1 * V1) / pow_d(H1, 0.33333);
+        R U2 = (QX1 - DTAU - S.DT * WSF * U1) / o.H;
+        R V2 = (QY1 - DTAV - S.DT * WSF * V1) / o.H;
+        const R cap = (VAR == 1) ? (R)5.0 : (R)15.0;
+        o.U = copysign(fmin(fabs(U2), cap), U2);
+        o.V = copysign(fmin(fabs(V2), cap), V2);
+    }
+    o.W = sqrt_r<R>(o.U * o.U + o.V * o.V);
+    return o;
+}
+
+// Accumulate one slot into the cell sums in the reference's order (F:732-742).
+template <class R>
+__host__ __device__ __forceinline__ void accumulate(R &WH, R &WU, R &WV, const Flux<R> &f, R SL, R SLCA,
+                                                    R SLSA) {
+    const R FLR_1 = f.f1 + f.f3;
+    const R FLR_2 = f.f2;
+    WH += SL * f.f0;
+    WU += SLCA * FLR_1 - SLSA * FLR_2;
+    WV += SLSA * FLR_1 + SLCA * FLR_2;
+}
+
+}  // namespace hydro

@@ -1,0 +1,127 @@
+"""ctypes binding of libhydro_amd.so (the C ABI in include/hydro.h).
+
+The shared library is built in-tree by ``make -C hydro-bench_amd`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
+is no fallback implementation of the timestep.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhydro_amd.so")
+
+ABI_VERSION = 1
+
+# hydro_status
+OK, ERR_INVALID, ERR_HIP, ERR_RANGE, ERR_UNSUPPORTED, ERR_NOMEM = range(6)
+# hydro_dtype
+F64, F32 = 0, 1
+# hydro_variant
+VARIANT_A, VARIANT_B = 0, 1
+# hydro_mode
+MODE_TWO_KERNEL, MODE_FUSED, MODE_GRAPH, MODE_PERSISTENT, MODE_GRAPH_TWO_KERNEL = range(5)
+MODES = {
+    "two_kernel": MODE_TWO_KERNEL,
+    "fused": MODE_FUSED,
+    "graph": MODE_GRAPH,
+    "persistent": MODE_PERSISTENT,
+    "graph_two_kernel": MODE_GRAPH_TWO_KERNEL,
+}
+# hydro_reorder
+REORDER_NONE, REORDER_RCM, REORDER_HILBERT = range(3)
+
+
+class HydroError(RuntimeError):
+    """A hydro_* call returned a non-zero hydro_status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"hydro status {status}: {message}")
+        self.status = status
+
+
+class MeshDesc(ctypes.Structure):
+    """hydro_mesh_desc (include/hydro.h)."""
+
+    _fields_ = [
+        ("abi_version", c_int32), ("dtype", c_int32), ("variant", c_int32), ("CELL", c_int32),
+        ("NDAYS", c_int32), ("NHQ", c_int32), ("MDT", c_int32), ("reserved0", c_int32),
+        ("DT", c_double), ("HM1", c_double), ("HM2", c_double), ("JL", c_double),
+        ("H", c_void_p), ("U", c_void_p), ("V", c_void_p), ("Z", c_void_p), ("W", c_void_p),
+        ("ZBC", c_void_p), ("ZB1", c_void_p), ("AREA", c_void_p), ("FNC", c_void_p),
+        ("BoundaryFeature", c_void_p), ("NHQ1", c_void_p),
+        ("ZW", c_void_p), ("QW", c_void_p),
+        ("QT", c_void_p), ("DQT", c_void_p), ("ZT", c_void_p), ("DZT", c_void_p),
+        ("NAC", c_void_p), ("KLAS", c_void_p),
+        ("SIDE", c_void_p), ("COSF", c_void_p), ("SINF", c_void_p), ("SLCOS", c_void_p), ("SLSIN", c_void_p),
+        ("FLUX0", c_void_p), ("FLUX1", c_void_p), ("FLUX2", c_void_p), ("FLUX3", c_void_p),
+    ]
+
+
+class Tune(ctypes.Structure):
+    """hydro_tune (include/hydro.h)."""
+
+    _fields_ = [
+        ("block", c_int32), ("reorder", c_int32), ("xcd_remap", c_int32), ("graph_chunk", c_int32),
+        ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("reserved", c_int32 * 2),
+    ]
+
+
+def default_tune(**kw) -> Tune:
+    t = Tune(block=256, reorder=REORDER_RCM, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
+             materialize_flux=0)
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+# Every symbol include/hydro.h declares, with (restype, argtypes).
+SIGNATURES = {
+    "hydro_mesh_create": (c_int, [POINTER(MeshDesc), POINTER(c_void_p)]),
+    "hydro_mesh_destroy": (c_int, [c_void_p]),
+    "hydro_prepare": (c_int, [c_void_p, POINTER(Tune)]),
+    "hydro_calculate_flux": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "hydro_update_cell": (c_int, [c_void_p, c_void_p]),
+    "hydro_step_n": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "hydro_set_time": (c_int, [c_void_p, c_int, c_int]),
+    "hydro_get_time": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+    "hydro_sync": (c_int, [c_void_p, c_void_p]),
+    "hydro_reload": (c_int, [c_void_p, c_void_p]),
+    "hydro_last_error": (c_char_p, []),
+    "hydro_abi_version": (c_int, []),
+    "hydro_dump": (c_int, [c_void_p, c_char_p, c_void_p]),
+    "hydro_set_profiling": (c_int, [c_void_p, c_int]),
+    "hydro_last_timing": (c_int, [c_void_p, POINTER(c_double), POINTER(c_int), POINTER(c_int)]),
+    "hydro_precalculate": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_double] + [c_void_p] * 9),
+    "hydro_rcm_order": (c_int, [c_int, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libhydro_amd.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C hydro-bench_amd` or __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.hydro_abi_version() != ABI_VERSION:
+        raise ImportError(f"libhydro_amd ABI {lib.hydro_abi_version()} != {ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = load().hydro_last_error()
+        raise HydroError(status, msg.decode() if msg else "")

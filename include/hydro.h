@@ -1,0 +1,210 @@
+/*
+ * hydro.h -- C ABI of the MI355X shallow-water timestep library
+ * (libhydro_amd.so, built from hydro-bench_amd/csrc).
+ *
+ * The library replaces the two-kernel timestep of hydro-cal as benchmarked by
+ * chhnb/hydro-bench:
+ *
+ *   void CalculateFlux(const MeshView&, int kt, int jt, cudaStream_t);
+ *   void UpdateCell(const MeshView&, cudaStream_t);
+ *        (cuda_native_impl/hydro-cal-src/include/functors.cuh:59-61,
+ *         bodies cuda_native_impl/hydro-cal-src/src/functors.cu:787-799)
+ *
+ * and the step loops that drive them
+ *        (cuda_native_impl/benchmark.cu:87-96 runSteps,
+ *         taichi_impl/F2_hydro_taichi_fp64.py:734-753 step_fn,
+ *         cuda_impl/F2_driver.cu:34-44 persistent driver,
+ *         cuda_native_impl/benchmark.cu:225-266 graph replay).
+ *
+ * Plain C types only: every array argument is a raw device pointer plus the
+ * sizes carried in hydro_mesh_desc.  No function calls exit(); each returns a
+ * hydro_status and hydro_last_error() gives the message (thread-local).
+ *
+ * Layout (the reference's external layout, kept as the drop-in contract;
+ * cuda_native_impl/hydro-cal-src/src/cell.cpp:11-147, side.cpp:11-52,
+ * hydro-cal/SOURCE/include/mesh/{cell,side,mesh}.hpp):
+ *   cell arrays    [CELL]            H U V Z W ZBC ZB1 AREA FNC BoundaryFeature NHQ1
+ *   slot arrays    [4*CELL]          slot idx = 4*cell + j, j = 0..3 (a triangle's
+ *                                    4th slot is padding with SIDE = 0)
+ *                                    NAC (1-based neighbour, 0 = none), KLAS,
+ *                                    SIDE COSF SINF SLCOS SLSIN FLUX0..3
+ *   time series    [NDAYS*CELL]      QT DQT ZT DZT, element [jt*CELL + cell]
+ *   rating curves  [CELL*NHQ]        ZW QW, element [cell*NHQ + k]
+ * Real arrays are double when dtype == HYDRO_F64 and float when HYDRO_F32.
+ * NAC and KLAS are int32 here; the CUDA reference stores them as Real, the
+ * Taichi port stores NAC as i32 (F2_hydro_taichi_fp64.py:46).  Their values
+ * are small integers, so every comparison the kernels make is unchanged.
+ *
+ * Threading: calls on one mesh are stream ordered and must not overlap
+ * (H, U, V, Z are updated in place and KLAS 7 slots are rewritten), exactly as
+ * for the reference.
+ */
+#ifndef HYDRO_AMD_H
+#define HYDRO_AMD_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYDRO_ABI_VERSION 1
+
+typedef enum hydro_status {
+    HYDRO_OK = 0,
+    HYDRO_ERR_INVALID = 1,     /* bad argument, shape or layout */
+    HYDRO_ERR_HIP = 2,         /* a HIP runtime call failed */
+    HYDRO_ERR_RANGE = 3,       /* step request beyond the NDAYS x (steps_per_day-1) cap */
+    HYDRO_ERR_UNSUPPORTED = 4, /* mode or option not available for this mesh */
+    HYDRO_ERR_NOMEM = 5
+} hydro_status;
+
+typedef enum hydro_dtype { HYDRO_F64 = 0, HYDRO_F32 = 1 } hydro_dtype;
+
+/* Variant A: cuda_native_impl/hydro-cal-src/src/functors.cu (default; the
+ *            Taichi ports are bit-exact with it at step 1).
+ * Variant B: cuda_native_impl/baselines/native_cuda_v0/fp64/hydro-cal-src/src/functors.cu
+ *            (KLAS 1 uses kt, supercritical test without H<HM2, per-cell CFL
+ *             DT2, +-5 m/s velocity cap). */
+typedef enum hydro_variant { HYDRO_VARIANT_A = 0, HYDRO_VARIANT_B = 1 } hydro_variant;
+
+/* Step execution modes for hydro_step_n. */
+typedef enum hydro_mode {
+    HYDRO_MODE_TWO_KERNEL = 0, /* calculate_flux + update_cell launches on the reference layout */
+    HYDRO_MODE_FUSED = 1,      /* one fused flux+update launch per step on the reordered layout */
+    HYDRO_MODE_GRAPH = 2,      /* FUSED steps captured once into a hipGraph and replayed */
+    HYDRO_MODE_PERSISTENT = 3, /* one launch for all steps, grid-wide barrier between steps */
+    HYDRO_MODE_GRAPH_TWO_KERNEL = 4 /* TWO_KERNEL steps captured into a hipGraph */
+} hydro_mode;
+
+/* Mesh descriptor: the reference MeshView (hydro-cal/SOURCE/include/mesh/mesh.hpp:114-142)
+ * flattened into plain pointers.  All pointers are DEVICE pointers owned by the
+ * caller and must stay valid until hydro_mesh_destroy.  Optional arrays may be
+ * NULL when the mesh has no slot of the kind that reads them (checked at
+ * create time): QT/DQT for KLAS 10, ZT/DZT for KLAS 1, ZW/QW/NHQ1 for KLAS 3,
+ * BoundaryFeature for KLAS 6/7. */
+typedef struct hydro_mesh_desc {
+    int32_t abi_version;   /* HYDRO_ABI_VERSION */
+    int32_t dtype;         /* hydro_dtype */
+    int32_t variant;       /* hydro_variant */
+    int32_t CELL;
+    int32_t NDAYS;
+    int32_t NHQ;
+    int32_t MDT;           /* seconds per boundary-series row; steps_per_day = (int)(MDT / DT) */
+    int32_t reserved0;
+    double DT, HM1, HM2, JL; /* converted to Real inside (float for HYDRO_F32) */
+
+    /* state [CELL], read and written */
+    void *H, *U, *V, *Z, *W;
+    /* cell statics [CELL] */
+    const void *ZBC, *ZB1, *AREA, *FNC, *BoundaryFeature;
+    const int32_t *NHQ1;
+    /* rating curves [CELL*NHQ] */
+    const void *ZW, *QW;
+    /* boundary series [NDAYS*CELL] */
+    const void *QT, *DQT, *ZT, *DZT;
+    /* slots [4*CELL] */
+    const int32_t *NAC;
+    int32_t *KLAS;         /* written: a KLAS 7 slot that overflows becomes 0 */
+    const void *SIDE, *COSF, *SINF, *SLCOS, *SLSIN;
+    void *FLUX0, *FLUX1, *FLUX2, *FLUX3; /* written by calculate_flux */
+} hydro_mesh_desc;
+
+typedef struct hydro_mesh hydro_mesh; /* opaque handle */
+
+/* Tuning of the reordered (FUSED / GRAPH / PERSISTENT) path. */
+typedef enum hydro_reorder { HYDRO_REORDER_NONE = 0, HYDRO_REORDER_RCM = 1, HYDRO_REORDER_HILBERT = 2 } hydro_reorder;
+
+typedef struct hydro_tune {
+    int32_t block;          /* threads per workgroup of the fused step (64..256, multiple of 64) */
+    int32_t reorder;        /* hydro_reorder */
+    int32_t xcd_remap;      /* 1: map workgroups to XCDs in contiguous tile ranges */
+    int32_t graph_chunk;    /* steps per captured hipGraph (GRAPH modes) */
+    int32_t persistent_wgs_per_cu; /* 0 = occupancy maximum */
+    int32_t materialize_flux; /* 1: FUSED modes also write FLUX0..3 every step */
+    int32_t reserved[2];
+} hydro_tune;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Validate the descriptor and build the handle.  Does not launch work; the
+ * caller's FLUX0..3 must already hold zeros, as cuda_native_impl side.cpp:208-211
+ * and F2_hydro_taichi_fp64.py:768-771 guarantee for the reference. */
+int hydro_mesh_create(const hydro_mesh_desc *desc, hydro_mesh **out);
+int hydro_mesh_destroy(hydro_mesh *m);
+
+/* Build the reordered internal layout used by the FUSED, GRAPH and
+ * PERSISTENT modes (host-side reorder + device upload, synchronous).
+ * tune may be NULL for defaults. */
+int hydro_prepare(hydro_mesh *m, const hydro_tune *tune);
+
+/* ---- the reference operator API ----------------------------------------- */
+
+/* One flux pass: replaces CalculateFlux(mesh, kt, jt, stream)
+ * (cuda_native_impl/hydro-cal-src/include/functors.cuh:59, functors.cu:787-792). */
+int hydro_calculate_flux(hydro_mesh *m, int kt, int jt, hipStream_t stream);
+
+/* One cell update: replaces UpdateCell(mesh, stream)
+ * (functors.cuh:61, functors.cu:794-799). */
+int hydro_update_cell(hydro_mesh *m, hipStream_t stream);
+
+/* ---- step loop ----------------------------------------------------------- */
+
+/* Run nsteps steps following the reference's (day, kt) order
+ * (benchmark.cu:87-96: for day < NDAYS, for kt = 1 .. steps_per_day-1)
+ * starting from the mesh's time cursor, which then advances.  Returns
+ * HYDRO_ERR_RANGE without running anything if fewer than nsteps steps remain
+ * (the reference's NDAYS cap, scripts/check_correctness.py:197-225). */
+int hydro_step_n(hydro_mesh *m, int nsteps, int mode, hipStream_t stream);
+
+int hydro_set_time(hydro_mesh *m, int day, int kt);
+int hydro_get_time(const hydro_mesh *m, int *day, int *kt, int *steps_per_day);
+
+/* After FUSED/GRAPH/PERSISTENT steps the current state lives in the reordered
+ * internal buffers.  hydro_sync writes it back to the caller's H U V Z W (and
+ * KLAS for rewritten KLAS 7 slots, and FLUX0..3 when materialize_flux is set).
+ * hydro_reload goes the other way, after the caller rewrote H U V Z in place.
+ * Both are stream ordered; switching modes calls them implicitly. */
+int hydro_sync(hydro_mesh *m, hipStream_t stream);
+int hydro_reload(hydro_mesh *m, hipStream_t stream);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+
+const char *hydro_last_error(void);
+int hydro_abi_version(void);
+
+/* Dump in the reference benchmark's binary format (benchmark.cu:142-161):
+ * int32 CELL; Real H,U,V,Z [CELL]; Real SLCOS,SLSIN,SIDE [4*CELL];
+ * Real FLUX0..3 [4*CELL].  Synchronizes the stream. */
+int hydro_dump(hydro_mesh *m, const char *path, hipStream_t stream);
+
+/* Per-launch timing of the dominant kernel over the last hydro_step_n call,
+ * measured with hipEvents on the launch stream (FUSED/GRAPH/PERSISTENT).
+ * Enabled by hydro_set_profiling(m, 1); adds one event pair per launch. */
+int hydro_set_profiling(hydro_mesh *m, int enable);
+int hydro_last_timing(const hydro_mesh *m, double *total_ms, int *launches, int *steps);
+
+/* ---- host-side mesh preparation ------------------------------------------ */
+
+/* Geometry preprocessing of MeshData::preCalculate
+ * (cuda_native_impl/hydro-cal-src/src/mesh.cpp:280-372), on HOST arrays, in
+ * Real = double (dtype HYDRO_F64) or float (HYDRO_F32; the fp32 reference
+ * runs this step in float too, taichi_impl/mesh_loader_f1.py:38-44).
+ *   NAP [4*CELL] 1-based node ids per cell (4th = 0 or = 1st for a triangle)
+ *   XP, YP [NOD] node coordinates, shifted in place by their minima
+ *   ZBC, FNC0 [CELL] bed and Manning n; Z [CELL] initial level, rewritten
+ *   for dry cells
+ * Outputs SIDE SINF COSF SLCOS SLSIN [4*CELL], AREA ZB1 H FNC [CELL]. */
+int hydro_precalculate(int dtype, int CELL, int NOD, const int32_t *NAP, void *XP, void *YP,
+                       const void *ZBC, void *Z, const void *FNC0, double HM1, void *SIDE, void *SINF,
+                       void *COSF, void *SLCOS, void *SLSIN, void *AREA, void *ZB1, void *H, void *FNC);
+
+/* Reverse Cuthill-McKee order of the cell graph of a host NAC array:
+ * perm[new] = old.  The reordered layout of hydro_prepare uses it. */
+int hydro_rcm_order(int CELL, const int32_t *NAC, int32_t *perm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYDRO_AMD_H */

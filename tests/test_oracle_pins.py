@@ -1,0 +1,339 @@
+"""Pins of the CPU oracle against what the mathematics fixes (no GPU).
+
+The reference ships no golden state vectors for this path (its dumps are
+missing blobs), so the oracle is pinned by checks a plausible restatement
+mistake would fail:
+
+* the Osher flux against an independent numerical evaluation of Osher's
+  path integral  F = f(qL) + int A^-(q) dq  along the P-ordered wave curves
+  (Gauss-Legendre on each curve), over random states covering all 16
+  (K1, K2) cases;
+* lake at rest: depth bitwise invariant after a step, velocities at rounding;
+* mass budget of a discharge (KLAS 10) inflow against the hydrograph;
+* uniform flow on a flat bed: the cell update reduces to explicit Manning
+  friction decay, and the rotated flux sums close to zero;
+* wall flux = hydrostatic pressure g h^2 / 2;
+* water-level boundary (KLAS 1) at equilibrium: zero discharge, and the
+  variant A/B difference (jt vs kt) shows where it must;
+* variant B's CFL step on an equilateral triangle (inradius closed form);
+* geometry (preCalculate): areas, outward unit normals, closed-polygon sums,
+  and bitwise agreement of the product's native routine with the oracle's;
+* fp32 oracle within the reference's own fp32-vs-fp64 envelope.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import cached_mesh, rel_err
+
+G = 9.81
+
+
+# ---------------------------------------------------------------------------
+# Osher flux by quadrature
+
+
+def _phys(h, u, v):
+    """Edge-normal physical flux: mass, normal momentum (with pressure), tangential momentum."""
+    return np.array([h * u, h * u * u + 0.5 * G * h * h, h * u * v])
+
+
+def _curve_integral(u0, u1, FI, v, family, npts=8):
+    """int over u in [u0,u1] of min(lambda,0) dq/du du along a Riemann-invariant curve.
+
+    family 1: u + 2c = FI (c = (FI-u)/2), lambda = u - c
+    family 2: u - 2c = FI (c = (u-FI)/2), lambda = u + c
+    The integrand is a polynomial of degree <= 3 in u, so Gauss-Legendre
+    with 8 nodes on each sub-interval of constant lambda sign is exact.
+    """
+    if u0 == u1:
+        return np.zeros(3)
+    # lambda is linear in u: lam = 1.5u - FI/2 for both families
+    u_sonic = FI / 3.0
+    pieces = [(u0, u1)]
+    if (u0 - u_sonic) * (u1 - u_sonic) < 0:
+        pieces = [(u0, u_sonic), (u_sonic, u1)]
+    xg, wg = np.polynomial.legendre.leggauss(npts)
+    total = np.zeros(3)
+    for a, b in pieces:
+        mid = 0.5 * (a + b)
+        lam_mid = 1.5 * mid - 0.5 * FI
+        if lam_mid >= 0:
+            continue
+        us = 0.5 * (b - a) * xg + mid
+        for uu, w in zip(us, wg):
+            if family == 1:
+                c = 0.5 * (FI - uu)
+                dh = -c / G
+            else:
+                c = 0.5 * (uu - FI)
+                dh = c / G
+            h = c * c / G
+            lam = 1.5 * uu - 0.5 * FI
+            dq = np.array([dh, h + uu * dh, v * dh])
+            total += 0.5 * (b - a) * w * lam * dq
+    return total
+
+
+def osher_quadrature(ql, qr):
+    hL, uL, vL = ql
+    hR, uR, vR = qr
+    cL, cR = math.sqrt(G * hL), math.sqrt(G * hR)
+    FIL, FIR = uL + 2 * cL, uR - 2 * cR
+    uA = 0.5 * (FIL + FIR)
+    cA = 0.25 * (FIL - FIR)
+    hA = cA * cA / G
+    F = _phys(hL, uL, vL)
+    F = F + _curve_integral(uL, uA, FIL, vL, 1)                # 1-curve, v = vL
+    F = F + min(uA, 0.0) * np.array([0.0, 0.0, hA * (vR - vL)])  # contact (shear) wave
+    F = F + _curve_integral(uA, uR, FIR, vR, 2)                # 2-curve, v = vR
+    return F, (FIL, FIR, uA, cA)
+
+
+def _cases(ql, qr):
+    hL, uL, _ = ql
+    hR, uR, _ = qr
+    cL, cR = math.sqrt(G * hL), math.sqrt(G * hR)
+    FIL, FIR = uL + 2 * cL, uR - 2 * cR
+    UA, CA = (FIL + FIR) / 2, abs((FIL - FIR) / 4)
+    K2 = 1 if CA < UA else 2 if (0 <= UA < CA) else 3 if (-CA <= UA < 0) else 4
+    rs = uR >= -cR
+    K1 = 1 if (uL < cL and rs) else 2 if (uL >= cL and rs) else 3 if (uL < cL and uR < -cR) else 4
+    return K1, K2
+
+
+def test_osher_against_path_integral(oracle):
+    rng = np.random.default_rng(11)
+    seen = {}
+    n_checked = 0
+    while n_checked < 4000:
+        hL, hR = rng.uniform(0.02, 6.0, 2)
+        uL, uR = rng.uniform(-14, 14, 2)
+        vL, vR = rng.uniform(-3, 3, 2)
+        ql, qr = (hL, uL, vL), (hR, uR, vR)
+        # the model's Riemann problem needs a physical (non-vacuum) star state
+        if (uL + 2 * math.sqrt(G * hL)) - (uR - 2 * math.sqrt(G * hR)) <= 0:
+            continue
+        K = _cases(ql, qr)
+        seen[K] = seen.get(K, 0) + 1
+        exact, (FIL, FIR, uA, cA) = osher_quadrature(ql, qr)
+        got = oracle.osher(ql, qr, uL + 2 * math.sqrt(G * hL), hL)
+        if K in ((3, 3), (4, 3)):
+            # Documented model deviation (MF:1288-1297 common-block FIR, kept by
+            # F:479 and T2): the right sonic point is evaluated after QS<5>
+            # replaced FIR by FIR - UA, i.e. at u = (FIR-UA)/3 instead of FIR/3.
+            us, us2 = FIR / 3.0, (FIR - uA) / 3.0
+            exact = exact + _phys(us * us / G, us, vR) - _phys(us2 * us2 / G, us2, vR)
+        mine = np.array([got[0], got[1] + got[3], got[2]])
+        scale = max(1.0, np.max(np.abs(exact)))
+        assert np.max(np.abs(mine - exact)) <= 1e-11 * scale, (K, ql, qr, mine, exact)
+        n_checked += 1
+    missing = [(a, b) for a in range(1, 5) for b in range(1, 5) if (a, b) not in seen]
+    assert not missing, f"cases never sampled: {missing}"
+
+
+def test_osher_consistency(oracle):
+    """Equal left/right states give the exact physical flux (all four regimes)."""
+    for h, u in [(1.0, 0.3), (1.0, -0.4), (0.5, 3.0), (0.5, -3.0), (2.0, 0.0)]:
+        v = 0.7
+        got = oracle.osher((h, u, v), (h, u, v), u + 2 * math.sqrt(G * h), h)
+        exp = _phys(h, u, v)
+        assert np.allclose([got[0], got[1] + got[3], got[2]], exp, rtol=1e-14, atol=1e-14)
+
+
+def test_osher_subsonic_pressure_split(oracle):
+    """Subsonic right-going case (K1=1, K2=2): F3 is g/2 * hA^2 of the star depth."""
+    ql, qr = (1.2, 0.2, 0.1), (1.0, 0.1, -0.2)
+    FIL = ql[1] + 2 * math.sqrt(G * ql[0])
+    FIR = qr[1] - 2 * math.sqrt(G * qr[0])
+    cA = (FIL - FIR) / 4
+    got = oracle.osher(ql, qr, FIL, ql[0])
+    assert abs(got[3] - 0.5 * G * (cA * cA / G) ** 2) <= 1e-14 * got[3]
+
+
+# ---------------------------------------------------------------------------
+# whole-step closed forms on the 6,675-cell channel
+
+
+def _walls_only(host):
+    h = host.copy()
+    k = h.arrays["KLAS"]
+    k[k != 0] = 4
+    return h
+
+
+def test_lake_at_rest(oracle):
+    host = _walls_only(cached_mesh("channel_6675"))
+    a = host.arrays
+    level = float(a["ZBC"].max() + 0.5)
+    a["Z"][:] = level
+    a["H"][:] = a["Z"] - a["ZBC"]
+    a["U"][:] = 0
+    a["V"][:] = 0
+    H0 = a["H"].copy()
+    r = oracle.OracleRun(host)
+    r.step(1)
+    assert np.array_equal(r["H"], H0)                      # depth bitwise invariant
+    assert np.array_equal(r["Z"], H0 + a["ZBC"])
+    assert np.all(r["FLUX0"][a["NAC"] > 0] == 0)           # no mass crosses any interior edge
+    assert np.abs(r["U"]).max() < 1e-13 and np.abs(r["V"]).max() < 1e-13
+    r.step(49)
+    assert np.abs(r["H"] - H0).max() < 1e-12
+    assert np.abs(r["U"]).max() < 1e-11
+
+
+def test_inflow_mass_budget(oracle):
+    """Outlet closed: the volume change of one step is DT times the
+    hydrograph discharge of the KLAS 10 set (interior edges cancel)."""
+    host = cached_mesh("channel_6675").copy()
+    k = host.arrays["KLAS"]
+    k[k == 1] = 4
+    r = oracle.OracleRun(host)
+    for step, (kt, jt) in enumerate([(1, 0), (2, 0), (30, 0), (1, 1)]):
+        r.day.value, r.kt.value = jt, kt
+        v0 = np.sum(r["H"] * host.AREA)
+        r.step(1)
+        v1 = np.sum(r["H"] * host.AREA)
+        # hydrograph at this step: rows at t = jt*MDT through float
+        # temporaries, linear in kt between rows (take_boundary_for_two_d)
+        n_in = int(np.count_nonzero((host.KLAS == 10) & (np.repeat(np.arange(host.CELL), 4) >= 0)
+                                    & (host.QT.reshape(host.NDAYS, host.CELL)[0][np.arange(4 * host.CELL) // 4] > 0)))
+        q = lambda t: 50.0 + 20.0 * math.sin(2 * math.pi * t / 3600.0)
+        K0 = host.steps_per_day
+        q0, q1 = q(jt * host.MDT), q((jt + 1) * host.MDT)
+        Q = q0 + (q1 - q0) * kt / K0
+        assert n_in > 0
+        assert abs((v1 - v0) / host.DT - Q) <= 2e-6 * Q, (kt, jt, (v1 - v0) / host.DT, Q)
+
+
+def test_uniform_flow_friction_decay(oracle):
+    """Flat bed, uniform depth and velocity: every interior cell's update is
+    explicit Manning friction, u' = u (1 - DT g n^2 |u| / h^(1+0.33333))."""
+    host = _walls_only(cached_mesh("channel_6675"))
+    a = host.arrays
+    a["ZBC"][:] = 10.0
+    a["ZB1"][:] = a["ZBC"] + host.HM1
+    a["H"][:] = 1.3
+    a["Z"][:] = a["ZBC"] + 1.3
+    u0, v0 = 0.8, -0.45
+    a["U"][:] = u0
+    a["V"][:] = v0
+    r = oracle.OracleRun(host)
+    r.step(1)
+    interior = np.all((host.NAC.reshape(-1, 4) > 0) | (host.SIDE.reshape(-1, 4) == 0), axis=1)
+    n = np.sqrt(host.FNC / 9.81)
+    decay = 1.0 - host.DT * G * n * n * math.hypot(u0, v0) / (1.3 * 1.3 ** 0.33333)
+    assert interior.sum() > 0.8 * host.CELL
+    assert np.allclose(r["U"][interior], u0 * decay[interior], rtol=1e-11, atol=1e-13)
+    assert np.allclose(r["V"][interior], v0 * decay[interior], rtol=1e-11, atol=1e-13)
+    assert np.allclose(r["H"][interior], 1.3, rtol=1e-13)
+
+
+def test_wall_flux_is_hydrostatic(oracle):
+    host = cached_mesh("channel_6675")
+    r = oracle.OracleRun(host)
+    r.calculate_flux(1, 0)
+    w = np.nonzero(host.KLAS == 4)[0]
+    h = host.H[w // 4]
+    assert np.all(r["FLUX0"][w] == 0) and np.all(r["FLUX1"][w] == 0) and np.all(r["FLUX2"][w] == 0)
+    assert np.allclose(r["FLUX3"][w], 0.5 * G * h * h, rtol=4e-16)
+
+
+def test_level_boundary_equilibrium_and_variants(oracle):
+    """KLAS 1 with the boundary level equal to the cell's level at rest:
+    no discharge in variant A (level uses jt = 0); variant B reads the
+    rising level at kt = 5 and takes water in."""
+    host = cached_mesh("channel_6675").copy()
+    a = host.arrays
+    a["U"][:] = 0
+    a["V"][:] = 0
+    s1 = np.nonzero(host.KLAS == 1)[0]
+    cells = np.unique(s1 // 4)
+    C = host.CELL
+    a["ZT"][cells] = host.Z[cells]        # row jt = 0
+    a["DZT"][cells] = 0.01
+    fl = {}
+    for var in (0, 1):
+        r = oracle.OracleRun(host, variant=var)
+        r.calculate_flux(5, 0)
+        fl[var] = r["FLUX0"][s1]
+    H = host.H[s1 // 4]
+    assert np.all(np.abs(fl[0]) <= 1e-12 * H)
+    assert np.all(fl[1] < 0)
+
+
+def test_variant_b_cfl_equilateral(oracle, tmp_path):
+    """Variant B's per-cell step uses the inradius a/(2 sqrt 3) of a triangle."""
+    import hydro_amd.mesh as M
+
+    a_len = 10.0
+    pts = np.array([[0, 0], [a_len, 0], [a_len / 2, a_len * math.sqrt(3) / 2]])
+    raw = dict(CELL=1, NOD=3, XP=pts[:, 0].copy(), YP=pts[:, 1].copy(),
+               NAP=np.array([1, 2, 3, 0], np.int32), NAC=np.zeros(4, np.int32),
+               KLAS=np.array([10, 4, 4, 0], np.int32), ZBC=np.array([5.0]), Zinit=np.array([6.0]),
+               n=np.array([0.03]), NDAYS=2, MDT=1000, DT=5.0, HM1=0.001, HM2=0.01,
+               q_series={0: np.array([2.0, 2.0])}, z_series={})
+    host = M._finish("tri", raw, np.float64)
+    r = oracle.OracleRun(host, variant=1)
+    r.step(1)
+    WH = float(np.sum(host.SIDE * r["FLUX0"]))
+    H1 = 1.0
+    dt2 = -(r["H"][0] - H1) * host.AREA[0] / WH
+    inr = a_len / (2 * math.sqrt(3))
+    assert abs(host.AREA[0] - math.sqrt(3) / 4 * a_len ** 2) < 1e-12
+    assert abs(dt2 - inr / (0 + math.sqrt(G * H1))) < 1e-9 * inr
+
+
+# ---------------------------------------------------------------------------
+# geometry
+
+
+def test_precalculate_product_matches_oracle_and_closed_forms(oracle):
+    host = cached_mesh("channel_6675")
+    ref = oracle.precalculate(host.CELL, len(host.XP), host.NAP, host.XP, host.YP, host.ZBC, host.Z, np.sqrt(host.FNC / 9.81), host.HM1)
+    for k in ("SIDE", "SINF", "COSF", "SLCOS", "SLSIN", "AREA", "ZB1"):
+        assert np.array_equal(host.arrays[k], ref[k]), k
+    # outward unit normals, perpendicular to their edge, closing every polygon
+    s = host.SIDE > 0
+    assert np.allclose(host.COSF[s] ** 2 + host.SINF[s] ** 2, 1.0, atol=1e-15)
+    assert np.allclose(host.SLCOS.reshape(-1, 4).sum(1), 0, atol=1e-9)
+    assert np.allclose(host.SLSIN.reshape(-1, 4).sum(1), 0, atol=1e-9)
+    # normals point away from the cell centre (after the origin shift)
+    NAP = host.NAP.reshape(-1, 4)
+    xc = np.array([host.XP[[n - 1 for n in row if n > 0]].mean() for row in NAP])
+    yc = np.array([host.YP[[n - 1 for n in row if n > 0]].mean() for row in NAP])
+    j = np.arange(4 * host.CELL)
+    n1 = NAP.ravel() - 1
+    nv = np.where(NAP[:, 3] > 0, 4, 3)
+    nx = np.where((j % 4) + 1 < nv[j // 4], j + 1, 4 * (j // 4))
+    n2 = NAP.ravel()[nx] - 1
+    mx = 0.5 * (host.XP[n1] + host.XP[n2])
+    my = 0.5 * (host.YP[n1] + host.YP[n2])
+    out = (mx - xc[j // 4]) * host.COSF + (my - yc[j // 4]) * host.SINF
+    assert np.all(out[s] > 0)
+    # areas: shoelace formula
+    area = np.zeros(host.CELL)
+    for i, row in enumerate(NAP):
+        ids = [n - 1 for n in row if n > 0]
+        x, y = host.XP[ids], host.YP[ids]
+        area[i] = 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+    assert np.allclose(host.AREA, area, rtol=1e-12)
+
+
+def test_fp32_oracle_within_reference_envelope(oracle):
+    """fp32 vs fp64 after 50 steps: inside the reference's own fp32 envelope
+    (1.1e-5 absolute on H at step 50, README.md:150-157), with margin x10."""
+    h64 = cached_mesh("channel_6675")
+    h32 = cached_mesh("channel_6675", dtype=np.float32)
+    r64, r32 = oracle.OracleRun(h64), oracle.OracleRun(h32)
+    r64.step(50)
+    r32.step(50)
+    assert np.max(np.abs(r32["H"].astype(np.float64) - r64["H"])) < 1.1e-4
+
+
+def test_oracle_step_cap(oracle):
+    host = cached_mesh("channel_6675")
+    r = oracle.OracleRun(host)
+    with pytest.raises(RuntimeError):
+        r.step(host.max_steps + 1)
