@@ -1,0 +1,307 @@
+// adjoint.hip -- tangent-linear and adjoint steps of the fp64 timestep on the
+// reordered layout (variant A).
+//
+// One adjoint step maps the adjoint of the state after a step (lam_out) to
+// the adjoint of the state before it (lam_in) and accumulates the gradients
+// of the per-cell parameters FNC (Manning, 9.81 n^2) and ZBC (bed).  The
+// forward step scatters nothing (each cell gathers its slot fluxes), so its
+// transpose is a gather too:
+//   k_adj_update  per cell: reverse of the cell update -> adjoints of the
+//                 cell's flux sums (WH, WU, WV) and of its old state;
+//   k_adj_flux    per cell: for each slot, the reverse of this cell's slot
+//                 flux (its own-state part) and of the neighbour's twin slot
+//                 flux (its neighbour-state part, i.e. this cell), seeded from
+//                 the flux-sum adjoints of the cell that owns each slot.
+// No atomics: every cell writes only its own adjoints, so results are
+// deterministic.
+//
+// k_tlm_step runs the forward step on dual numbers (dual.hpp): the same
+// arithmetic plus one directional derivative, used to check the adjoint.
+#include "adjoint.hpp"
+#include "dual.hpp"
+#include "kernels.hpp"
+
+namespace hydro {
+
+template <> struct StorageOf<ad::Dual> {
+    using type = double;
+};
+
+namespace {
+
+inline int div_up(int a, int b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ BcCtx<double> bc_of(const ExtView<double> &m, int cell, int slot, int kt, int jt) {
+    BcCtx<double> bc;
+    bc.QT = m.QT; bc.DQT = m.DQT; bc.ZT = m.ZT; bc.DZT = m.DZT;
+    bc.ZW = m.ZW; bc.QW = m.QW; bc.BF = m.BF; bc.NHQ1 = m.NHQ1;
+    bc.CELL = m.CELL; bc.NHQ = m.NHQ; bc.cell = cell; bc.kt = kt; bc.jt = jt;
+    bc.SIDE = m.SIDE[slot]; bc.COSF = m.COSF[slot]; bc.SINF = m.SINF[slot];
+    return bc;
+}
+
+__device__ __forceinline__ BcCtx<double> bc_none() {
+    BcCtx<double> bc;
+    bc.QT = bc.DQT = bc.ZT = bc.DZT = bc.ZW = bc.QW = bc.BF = nullptr;
+    bc.NHQ1 = nullptr;
+    bc.CELL = bc.NHQ = bc.cell = bc.kt = bc.jt = 0;
+    bc.SIDE = bc.COSF = bc.SINF = 0;
+    return bc;
+}
+
+__global__ void k_adj_update(FusedView<double> f, const State<double> *s_in, const double *wsum,
+                             const State<double> *lam_out, State<double> *lam_in, double *wbar, double *g_fnc,
+                             double *g_zbc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = f.C;
+    if (p >= C) return;
+    const State<double> s = s_in[p];
+    const State<double> l = lam_out[p];
+    const UpdateBar b = update_bar(f.S, s.H, s.U, s.V, f.AREA[p], f.FNC[p], wsum[p], wsum[C + p], wsum[2 * C + p],
+                                   l.H, l.U, l.V, l.Z);
+    wbar[p] = b.WH;
+    wbar[C + p] = b.WU;
+    wbar[2 * C + p] = b.WV;
+    lam_in[p] = State<double>{b.H1, b.U1, b.V1, 0.0};
+    g_fnc[p] += b.FNC;
+    g_zbc[p] += b.ZBC;
+}
+
+__device__ __forceinline__ UpdateBar wbar_of(const double *wbar, int C, int q) {
+    UpdateBar u;
+    u.WH = wbar[q];
+    u.WU = wbar[C + q];
+    u.WV = wbar[2 * C + q];
+    return u;
+}
+
+__global__ void k_adj_flux(FusedView<double> f, const uint8_t *twin, const State<double> *s_in,
+                           const double *wbar, State<double> *lam_in, double *g_zbc, int kt, int jt, int *unsupported) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = f.C;
+    if (p >= C) return;
+    const State<double> sp = s_in[p];
+    const SideIn me{sp.H, sp.U, sp.V, sp.Z, f.ZBC[p]};
+    const UpdateBar wp = wbar_of(wbar, C, p);
+    State<double> lam = lam_in[p];
+    double gz = 0;
+    bool bad = false;
+#pragma unroll 1
+    for (int j = 0; j < 4; j++) {
+        const int e = j * C + p;
+        const int kind = f.meta[e] & META_KIND_MASK;
+        if (kind == K_PAD) continue;
+        const int q = f.nbr[e];
+        const bool has = q >= 0;
+        SideIn nb{0, 0, 0, 0, 0};
+        if (has) {
+            const State<double> sq = s_in[q];
+            nb = SideIn{sq.H, sq.U, sq.V, sq.Z, f.ZBC[q]};
+        }
+        const double c = f.COSF[e], s = f.SINF[e], SL = f.SIDE[e];
+        const double SLCA = f.SLCOS ? f.SLCOS[e] : SL * c;
+        const double SLSA = f.SLSIN ? f.SLSIN[e] : SL * s;
+        double Fb[4];
+        flux_bar_of_sums(wp, SL, SLCA, SLSA, Fb);
+        const BcCtx<double> bc = (kind >= K_B10) ? bc_of(*f.ext, f.orig[p], 4 * f.orig[p] + j, kt, jt) : bc_none();
+        SideBar mb, nbb;
+        bad |= !slot_flux_bar(kind, f.S, bc, me, has, nb, c, s, Fb, mb, nbb);
+        lam.H += mb.H;
+        lam.U += mb.U;
+        lam.V += mb.V;
+        lam.Z += mb.Z;
+        gz += mb.ZB;
+        if (!has) continue;
+        // the neighbour's slot facing this cell: its flux depends on this cell
+        const int j2 = twin[e];
+        if (j2 > 3) continue;
+        const int e2 = j2 * C + q;
+        const int kind2 = f.meta[e2] & META_KIND_MASK;
+        if (kind2 == K_PAD) continue;
+        const double c2 = f.COSF[e2], s2 = f.SINF[e2], SL2 = f.SIDE[e2];
+        const double SLCA2 = f.SLCOS ? f.SLCOS[e2] : SL2 * c2;
+        const double SLSA2 = f.SLSIN ? f.SLSIN[e2] : SL2 * s2;
+        double Fb2[4];
+        flux_bar_of_sums(wbar_of(wbar, C, q), SL2, SLCA2, SLSA2, Fb2);
+        const BcCtx<double> bc2 =
+            (kind2 >= K_B10) ? bc_of(*f.ext, f.orig[q], 4 * f.orig[q] + j2, kt, jt) : bc_none();
+        SideBar mb2, nbb2;
+        bad |= !slot_flux_bar(kind2, f.S, bc2, nb, true, me, c2, s2, Fb2, mb2, nbb2);
+        lam.H += nbb2.H;
+        lam.U += nbb2.U;
+        lam.V += nbb2.V;
+        lam.Z += nbb2.Z;
+        gz += nbb2.ZB;
+    }
+    lam_in[p] = lam;
+    g_zbc[p] += gz;
+    if (bad) *unsupported = 1;
+}
+
+// Tangent-linear step: the forward step on dual numbers.
+__global__ void k_tlm_step(FusedView<double> f, const State<double> *s_in, const State<double> *ds_in,
+                           const double *dfnc, const double *dzbc, State<double> *s_out, State<double> *ds_out,
+                           int kt, int jt) {
+    using ad::Dual;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = f.C;
+    if (p >= C) return;
+    Scalars<Dual> S;
+    S.DT = f.S.DT;
+    S.HM1 = f.S.HM1;
+    S.HM2 = f.S.HM2;
+    S.JL = f.S.JL;
+    const State<double> a = s_in[p], da = ds_in[p];
+    const Dual H1(a.H, da.H), U1(a.U, da.U), V1(a.V, da.V), Z1(a.Z, da.Z);
+    const Dual ZBp(f.ZBC[p], dzbc[p]);
+    const Dual ZB1 = ZBp + f.S.HM1;
+    Dual WH = 0.0, WU = 0.0, WV = 0.0;
+    Dual sd[4];
+    for (int j = 0; j < 4; j++) {
+        const int e = j * C + p;
+        const int kind = f.meta[e] & META_KIND_MASK;
+        const double SL = f.SIDE[e];
+        sd[j] = SL;
+        Flux<Dual> fl;
+        bool fired = false;
+        if (kind == K_PAD) {
+            fl = slot_flux<Dual, 0>(K_MIRROR, S, BcCtx<Dual>{}, H1, U1, V1, Z1, ZB1, ZBp, false, 0.0, 0.0, 0.0,
+                                    0.0, 0.0, Dual(0.0), Dual(0.0), Flux<Dual>{0.0, 0.0, 0.0, 0.0}, &fired);
+            accumulate<Dual>(WH, WU, WV, fl, Dual(0.0), Dual(0.0), Dual(0.0));
+            continue;
+        }
+        const int q = f.nbr[e];
+        const bool has = q >= 0;
+        Dual Hn = 0.0, Un = 0.0, Vn = 0.0, Zn = 0.0, ZBn = 0.0;
+        if (has) {
+            const State<double> b = s_in[q], db = ds_in[q];
+            Hn = Dual(b.H, db.H);
+            Un = Dual(b.U, db.U);
+            Vn = Dual(b.V, db.V);
+            Zn = Dual(b.Z, db.Z);
+            ZBn = Dual(f.ZBC[q], dzbc[q]);
+        }
+        const double c = f.COSF[e], s = f.SINF[e];
+        BcCtx<Dual> bc{};
+        if (kind >= K_B10) {
+            const ExtView<double> &m = *f.ext;
+            const int oc = f.orig[p], eo = 4 * oc + j;
+            bc.QT = m.QT; bc.DQT = m.DQT; bc.ZT = m.ZT; bc.DZT = m.DZT;
+            bc.ZW = m.ZW; bc.QW = m.QW; bc.BF = m.BF; bc.NHQ1 = m.NHQ1;
+            bc.CELL = m.CELL; bc.NHQ = m.NHQ; bc.cell = oc; bc.kt = kt; bc.jt = jt;
+            bc.SIDE = m.SIDE[eo]; bc.COSF = m.COSF[eo]; bc.SINF = m.SINF[eo];
+        }
+        fl = slot_flux<Dual, 0>(kind, S, bc, H1, U1, V1, Z1, ZB1, ZBp, has, Hn, Un, Vn, Zn, ZBn, Dual(c), Dual(s),
+                                Flux<Dual>{0.0, 0.0, 0.0, 0.0}, &fired);
+        const double SLCA = f.SLCOS ? f.SLCOS[e] : SL * c;
+        const double SLSA = f.SLSIN ? f.SLSIN[e] : SL * s;
+        accumulate<Dual>(WH, WU, WV, fl, Dual(SL), Dual(SLCA), Dual(SLSA));
+    }
+    const Dual FNC(f.FNC[p], dfnc[p]);
+    const CellOut<Dual> o = update_math<Dual, 0>(S, H1, U1, V1, ZBp, Dual(f.AREA[p]), FNC, WH, WU, WV, sd[0], sd[1],
+                                                 sd[2], sd[3]);
+    s_out[p] = State<double>{o.H.v, o.U.v, o.V.v, o.Z.v};
+    ds_out[p] = State<double>{o.H.d, o.U.d, o.V.d, o.Z.d};
+}
+
+// gauges: value of Z (or add a seed into lam.Z) at ORIGINAL cell ids
+__global__ void k_gauge_gather(const State<double> *st, const int32_t *inv, const int32_t *cells, int n, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = st[inv[cells[i]]].Z;
+}
+
+__global__ void k_gauge_seed(State<double> *lam, const int32_t *inv, const int32_t *cells, int n, const double *seed) {
+    // one thread walks the list so repeated gauge ids accumulate deterministically
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int i = 0; i < n; i++) lam[inv[cells[i]]].Z += seed[i];
+}
+
+__global__ void k_to_external(const double *in, const int32_t *orig, double *out, int C) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < C) out[orig[p]] = in[p];
+}
+
+__global__ void k_to_internal(const double *in, const int32_t *orig, double *out, int C) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < C) out[p] = in[orig[p]];
+}
+
+__global__ void k_states_to_external(const State<double> *in, const int32_t *orig, double *H, double *U, double *V,
+                                     double *Z, int C) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= C) return;
+    const State<double> s = in[p];
+    const int o = orig[p];
+    H[o] = s.H;
+    U[o] = s.U;
+    V[o] = s.V;
+    Z[o] = s.Z;
+}
+
+__global__ void k_states_to_internal(const double *H, const double *U, const double *V, const double *Z,
+                                     const int32_t *orig, State<double> *out, int C) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= C) return;
+    const int o = orig[p];
+    out[p] = State<double>{H[o], U[o], V[o], Z[o]};
+}
+
+}  // namespace
+
+hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, const State<double> *s_in,
+                               const double *wsum, const State<double> *lam_out, State<double> *lam_in, double *wbar,
+                               double *g_fnc, double *g_zbc, int kt, int jt, int *unsupported, hipStream_t s) {
+    if (f.C == 0) return hipSuccess;
+    const int B = 256;
+    hipLaunchKernelGGL(k_adj_update, dim3(div_up(f.C, B)), dim3(B), 0, s, f, s_in, wsum, lam_out, lam_in, wbar, g_fnc,
+                       g_zbc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adj_flux, dim3(div_up(f.C, B)), dim3(B), 0, s, f, twin, s_in, wbar, lam_in, g_zbc, kt, jt,
+                       unsupported);
+    return hipGetLastError();
+}
+
+hipError_t launch_tlm_step(const FusedView<double> &f, const State<double> *s_in, const State<double> *ds_in,
+                           const double *dfnc, const double *dzbc, State<double> *s_out, State<double> *ds_out, int kt,
+                           int jt, hipStream_t s) {
+    if (f.C == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tlm_step, dim3(div_up(f.C, 128)), dim3(128), 0, s, f, s_in, ds_in, dfnc, dzbc, s_out, ds_out,
+                       kt, jt);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauge_gather(const State<double> *st, const int32_t *inv, const int32_t *cells, int n, double *out,
+                               hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gauge_gather, dim3(div_up(n, 256)), dim3(256), 0, s, st, inv, cells, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauge_seed(State<double> *lam, const int32_t *inv, const int32_t *cells, int n, const double *seed,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gauge_seed, dim3(1), dim3(64), 0, s, lam, inv, cells, n, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_map(const double *in, const int32_t *orig, double *out, int C, int to_external, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    if (to_external)
+        hipLaunchKernelGGL(k_to_external, dim3(div_up(C, 256)), dim3(256), 0, s, in, orig, out, C);
+    else
+        hipLaunchKernelGGL(k_to_internal, dim3(div_up(C, 256)), dim3(256), 0, s, in, orig, out, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_states_map(State<double> *st, const int32_t *orig, double *H, double *U, double *V, double *Z, int C,
+                             int to_external, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    if (to_external)
+        hipLaunchKernelGGL(k_states_to_external, dim3(div_up(C, 256)), dim3(256), 0, s, st, orig, H, U, V, Z, C);
+    else
+        hipLaunchKernelGGL(k_states_to_internal, dim3(div_up(C, 256)), dim3(256), 0, s, H, U, V, Z, orig, st, C);
+    return hipGetLastError();
+}
+
+}  // namespace hydro

@@ -104,7 +104,8 @@ struct hydro_mesh {
     bool prepared = false;
     int cur = 0;  // state record buffer holding the current state
     bool ext_valid = true, int_valid = false;
-    DevBuf st[2], W, ZBC, AREA, FNC, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig;
+    DevBuf st[2], W, ZBC, AREA, FNC, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
+    DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
     DevBuf tick, barrier, perr, ext_dev;
     bool need_zb1 = false, need_slcs = false;
     hipStream_t cap_stream = nullptr;
@@ -172,6 +173,8 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.SLSIN = m->need_slcs ? m->SLSIN.as<R>() : nullptr;
     f.orig = m->orig.as<int32_t>();
     f.materialize = m->tune.materialize_flux;
+    f.rec_state = nullptr;
+    f.rec_wsum = nullptr;
     return f;
 }
 
@@ -233,6 +236,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
     std::vector<int32_t> nbr_n(S4), orig_n(C);
     std::vector<uint8_t> meta_n(S4);
     std::vector<R> cos_n(S4), sin_n(S4), side_n(S4), slc_n(m->need_slcs ? S4 : 0), sls_n(m->need_slcs ? S4 : 0);
+    std::vector<uint8_t> twin_n(S4, 0xff);
     for (int p = 0; p < C; p++) {
         const int o = perm[p];
         orig_n[p] = o;
@@ -254,6 +258,12 @@ template <class R> int prepare_t(hydro_mesh *m) {
             int kind = pad ? K_PAD : kind_of_klas(KLAS[so], prim);
             meta_n[sn] = (uint8_t)(kind | (prim ? META_PRIMARY_BIT : 0));
             nbr_n[sn] = (NC >= 0) ? inv[NC] : -1;
+            if (NC >= 0)
+                for (int j2 = 0; j2 < 4; j2++)
+                    if (NAC[4 * (size_t)NC + j2] - 1 == o) {
+                        twin_n[sn] = (uint8_t)j2;
+                        break;
+                    }
             cos_n[sn] = COSF[so];
             sin_n[sn] = SINF[so];
             side_n[sn] = SIDE[so];
@@ -266,7 +276,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
     if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
         (rc = h2d(m->ZB1, zb1_n)) || (rc = h2d(m->nbr, nbr_n)) || (rc = h2d(m->meta, meta_n)) ||
         (rc = h2d(m->COSF, cos_n)) || (rc = h2d(m->SINF, sin_n)) || (rc = h2d(m->SIDE, side_n)) ||
-        (rc = h2d(m->SLCOS, slc_n)) || (rc = h2d(m->SLSIN, sls_n)) || (rc = h2d(m->orig, orig_n)))
+        (rc = h2d(m->SLCOS, slc_n)) || (rc = h2d(m->SLSIN, sls_n)) || (rc = h2d(m->orig, orig_n)) ||
+        (rc = h2d(m->inv, inv)) || (rc = h2d(m->twin, twin_n)))
         return rc;
     {
         const ExtView<R> ev = ext_view<R>(m);
@@ -697,6 +708,172 @@ int hydro_last_timing(const hydro_mesh *m, double *total_ms, int *launches, int 
     if (total_ms) *total_ms = m->last_ms;
     if (launches) *launches = m->last_launches;
     if (steps) *steps = m->last_steps;
+    return HYDRO_OK;
+}
+
+// ---- adjoint / tangent-linear support ---------------------------------------
+
+static int adj_ready(hydro_mesh *m) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (m->d.dtype != HYDRO_F64 || m->d.variant != HYDRO_VARIANT_A)
+        return fail(HYDRO_ERR_UNSUPPORTED, "adjoint and tangent-linear steps need an fp64 variant-A mesh");
+    if (!m->prepared) {
+        const hydro_tune t = m->tune;
+        int rc = hydro_prepare(m, &t);
+        if (rc) return rc;
+    }
+    if (m->need_zb1)
+        return fail(HYDRO_ERR_UNSUPPORTED, "adjoint needs ZB1 == ZBC + HM1 (bed gradients flow through ZB1)");
+    return HYDRO_OK;
+}
+
+int hydro_adj_sizes(const hydro_mesh *m, size_t *state_bytes, size_t *wsum_bytes, size_t *cellvec_bytes) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    const size_t C = (size_t)m->CELL;
+    const size_t rs = (size_t)m->rsize;
+    if (state_bytes) *state_bytes = C * 4 * rs;
+    if (wsum_bytes) *wsum_bytes = C * 3 * rs;
+    if (cellvec_bytes) *cellvec_bytes = C * rs;
+    return HYDRO_OK;
+}
+
+int hydro_state_save(hydro_mesh *m, void *state, hipStream_t s) {
+    if (!m || !state) return fail(HYDRO_ERR_INVALID, "null argument");
+    int rc = (m->d.dtype == HYDRO_F64) ? ensure_internal<double>(m, s) : ensure_internal<float>(m, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(state, m->st[m->cur].p, m->st[m->cur].bytes, hipMemcpyDeviceToDevice, s));
+    return HYDRO_OK;
+}
+
+int hydro_state_load(hydro_mesh *m, const void *state, hipStream_t s) {
+    if (!m || !state) return fail(HYDRO_ERR_INVALID, "null argument");
+    int rc = (m->d.dtype == HYDRO_F64) ? ensure_internal<double>(m, s) : ensure_internal<float>(m, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(m->st[m->cur].p, state, m->st[m->cur].bytes, hipMemcpyDeviceToDevice, s));
+    m->int_valid = true;
+    m->ext_valid = false;
+    return HYDRO_OK;
+}
+
+extern "C++" template <class R, int VAR> static int record_t(hydro_mesh *m, int n, void *states, void *wsums, hipStream_t s) {
+    int rc = ensure_internal<R>(m, s);
+    if (rc) return rc;
+    const size_t C = (size_t)m->CELL;
+    for (int i = 0; i < n; i++) {
+        FusedView<R> f = fused_view<R>(m, m->cur);
+        f.rec_state = (State<R> *)states + (size_t)i * C;
+        f.rec_wsum = (R *)wsums + (size_t)i * 3 * C;
+        HIP_TRY((launch_fused_step<R, VAR>(f, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
+        m->cur ^= 1;
+        advance(m);
+    }
+    m->ext_valid = false;
+    return HYDRO_OK;
+}
+
+int hydro_step_record(hydro_mesh *m, int nsteps, void *states, void *wsums, hipStream_t s) {
+    if (!m || !states || !wsums || nsteps < 0) return fail(HYDRO_ERR_INVALID, "bad argument");
+    const int per = m->spd - 1;
+    const int64_t remaining = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
+    if (nsteps > remaining) return fail(HYDRO_ERR_RANGE, "%d steps requested, %lld remain", nsteps, (long long)remaining);
+    if (!m->prepared) {
+        const hydro_tune t = m->tune;
+        int rc = hydro_prepare(m, &t);
+        if (rc) return rc;
+    }
+    if (m->d.dtype == HYDRO_F64)
+        return m->d.variant ? record_t<double, 1>(m, nsteps, states, wsums, s) : record_t<double, 0>(m, nsteps, states, wsums, s);
+    return m->d.variant ? record_t<float, 1>(m, nsteps, states, wsums, s) : record_t<float, 0>(m, nsteps, states, wsums, s);
+}
+
+int hydro_adjoint_step(hydro_mesh *m, int kt, int jt, const void *s_in, const void *wsum, const void *lam_out,
+                       void *lam_in, void *g_fnc, void *g_zbc, hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc) return rc;
+    if (!s_in || !wsum || !lam_out || !lam_in || !g_fnc || !g_zbc) return fail(HYDRO_ERR_INVALID, "null buffer");
+    if (jt < 0 || jt >= m->d.NDAYS) return fail(HYDRO_ERR_RANGE, "jt %d", jt);
+    if (!m->wbar.p && (rc = m->wbar.alloc((size_t)m->CELL * 3 * sizeof(double)))) return rc;
+    if (!m->adj_flag.p) {
+        if ((rc = m->adj_flag.alloc(sizeof(int)))) return rc;
+        HIP_TRY(hipMemset(m->adj_flag.p, 0, sizeof(int)));
+    }
+    const FusedView<double> f = fused_view<double>(m, m->cur);
+    HIP_TRY(launch_adjoint_step(f, m->twin.as<uint8_t>(), (const State<double> *)s_in, (const double *)wsum,
+                                (const State<double> *)lam_out, (State<double> *)lam_in, m->wbar.as<double>(),
+                                (double *)g_fnc, (double *)g_zbc, kt, jt, m->adj_flag.as<int>(), s));
+    return HYDRO_OK;
+}
+
+int hydro_adjoint_check(hydro_mesh *m, int *unsupported) {
+    if (!m || !unsupported) return fail(HYDRO_ERR_INVALID, "null argument");
+    *unsupported = 0;
+    if (m->adj_flag.p) HIP_TRY(hipMemcpy(unsupported, m->adj_flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    return HYDRO_OK;
+}
+
+int hydro_tlm_step(hydro_mesh *m, int kt, int jt, const void *s_in, const void *ds_in, const void *dfnc,
+                   const void *dzbc, void *s_out, void *ds_out, hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc) return rc;
+    if (!s_in || !ds_in || !dfnc || !dzbc || !s_out || !ds_out) return fail(HYDRO_ERR_INVALID, "null buffer");
+    if (jt < 0 || jt >= m->d.NDAYS) return fail(HYDRO_ERR_RANGE, "jt %d", jt);
+    const FusedView<double> f = fused_view<double>(m, m->cur);
+    HIP_TRY(launch_tlm_step(f, (const State<double> *)s_in, (const State<double> *)ds_in, (const double *)dfnc,
+                            (const double *)dzbc, (State<double> *)s_out, (State<double> *)ds_out, kt, jt, s));
+    return HYDRO_OK;
+}
+
+static int check_gauges(hydro_mesh *m, const int32_t *cells, int n) {
+    if (n < 0 || (n > 0 && !cells)) return fail(HYDRO_ERR_INVALID, "bad gauge list");
+    std::vector<int32_t> h;
+    int rc = d2h(h, cells, (size_t)n);
+    if (rc) return rc;
+    for (int i = 0; i < n; i++)
+        if (h[i] < 0 || h[i] >= m->CELL) return fail(HYDRO_ERR_INVALID, "gauge cell %d out of range", h[i]);
+    return HYDRO_OK;
+}
+
+int hydro_gauge_values(hydro_mesh *m, const void *state, const int32_t *cells, int n, double *out, hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc || (rc = check_gauges(m, cells, n))) return rc;
+    HIP_TRY(launch_gauge_gather((const State<double> *)state, m->inv.as<int32_t>(), cells, n, out, s));
+    return HYDRO_OK;
+}
+
+int hydro_gauge_seed(hydro_mesh *m, void *lam, const int32_t *cells, int n, const double *seed, hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc || (rc = check_gauges(m, cells, n))) return rc;
+    HIP_TRY(launch_gauge_seed((State<double> *)lam, m->inv.as<int32_t>(), cells, n, seed, s));
+    return HYDRO_OK;
+}
+
+int hydro_cells_map(hydro_mesh *m, const double *in, double *out, int to_external, hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc) return rc;
+    HIP_TRY(launch_cells_map(in, m->orig.as<int32_t>(), out, m->CELL, to_external, s));
+    return HYDRO_OK;
+}
+
+int hydro_states_map(hydro_mesh *m, void *state, double *H, double *U, double *V, double *Z, int to_external,
+                     hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc) return rc;
+    HIP_TRY(launch_states_map((State<double> *)state, m->orig.as<int32_t>(), H, U, V, Z, m->CELL, to_external, s));
+    return HYDRO_OK;
+}
+
+int hydro_refresh_statics(hydro_mesh *m, hipStream_t s) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (!m->prepared) return HYDRO_OK;
+    if (m->d.dtype == HYDRO_F64) {
+        HIP_TRY(launch_refresh_statics<double>(ext_view<double>(m), m->orig.as<int32_t>(), m->ZBC.as<double>(),
+                                               m->AREA.as<double>(), m->FNC.as<double>(),
+                                               m->need_zb1 ? m->ZB1.as<double>() : nullptr, m->CELL, s));
+    } else {
+        HIP_TRY(launch_refresh_statics<float>(ext_view<float>(m), m->orig.as<int32_t>(), m->ZBC.as<float>(),
+                                              m->AREA.as<float>(), m->FNC.as<float>(),
+                                              m->need_zb1 ? m->ZB1.as<float>() : nullptr, m->CELL, s));
+    }
     return HYDRO_OK;
 }
 

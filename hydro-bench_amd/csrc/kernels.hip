@@ -165,6 +165,12 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
             m.F0[eo] = fl.f0; m.F1[eo] = fl.f1; m.F2[eo] = fl.f2; m.F3[eo] = fl.f3;
         }
     }
+    if (f.rec_wsum) {
+        f.rec_state[p] = me;
+        f.rec_wsum[p] = WH;
+        f.rec_wsum[C + p] = WU;
+        f.rec_wsum[2 * C + p] = WV;
+    }
     const CellOut<R> o = update_math<R, VAR>(S, me.H, me.U, me.V, ZBp, f.AREA[p], f.FNC[p], WH, WU, WV,
                                              sd[0], sd[1], sd[2], sd[3]);
     Sout[p] = State<R>{o.H, o.U, o.V, o.Z};
@@ -288,9 +294,32 @@ __global__ void k_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uin
     meta[e] = (uint8_t)(kind_of_klas(KLAS_ext[4 * orig[p] + j], prim) | (prim ? META_PRIMARY_BIT : 0));
 }
 
+template <class R>
+__global__ void k_refresh_statics(ExtView<R> m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *ZB1, int C) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= C) return;
+    const int o = orig[p];
+    ZBC[p] = m.ZBC[o];
+    AREA[p] = m.AREA[o];
+    FNC[p] = m.FNC[o];
+    if (ZB1) ZB1[p] = m.ZB1[o];
+}
+
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace
+
+template <class R>
+hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *ZB1, int C,
+                                  hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_refresh_statics<R>), dim3(div_up(C, 256)), dim3(256), 0, s, m, orig, ZBC, AREA, FNC, ZB1, C);
+    return hipGetLastError();
+}
+template hipError_t launch_refresh_statics<double>(const ExtView<double> &, const int32_t *, double *, double *,
+                                                   double *, double *, int, hipStream_t);
+template hipError_t launch_refresh_statics<float>(const ExtView<float> &, const int32_t *, float *, float *, float *,
+                                                  float *, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Launch helpers.

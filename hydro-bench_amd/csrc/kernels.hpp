@@ -47,6 +47,10 @@ template <class R> struct FusedView {
     const R *COSF, *SINF, *SIDE;    // [4][C]
     const R *SLCOS, *SLSIN;    // [4][C] or null when SIDE*COSF / SIDE*SINF reproduce them
     const int32_t *orig;       // [C]
+    // recording for the adjoint (null when not recording): the step's input
+    // state, and its flux sums WH, WU, WV as [3][C]
+    State<R> *rec_state;
+    R *rec_wsum;
 };
 
 constexpr int kMaxBlock = 1024;
@@ -86,5 +90,24 @@ hipError_t launch_scatter(const ExtView<R> &m, const int32_t *orig, const State<
                           int C, hipStream_t s);
 hipError_t launch_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uint8_t *meta, int C,
                                hipStream_t s);
+// internal statics <- the caller's ZBC AREA FNC (and ZB1 when stored)
+template <class R>
+hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *ZB1, int C,
+                                  hipStream_t s);
+
+// adjoint.hip (fp64, variant A)
+hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, const State<double> *s_in,
+                               const double *wsum, const State<double> *lam_out, State<double> *lam_in, double *wbar,
+                               double *g_fnc, double *g_zbc, int kt, int jt, int *unsupported, hipStream_t s);
+hipError_t launch_tlm_step(const FusedView<double> &f, const State<double> *s_in, const State<double> *ds_in,
+                           const double *dfnc, const double *dzbc, State<double> *s_out, State<double> *ds_out, int kt,
+                           int jt, hipStream_t s);
+hipError_t launch_gauge_gather(const State<double> *st, const int32_t *inv, const int32_t *cells, int n, double *out,
+                               hipStream_t s);
+hipError_t launch_gauge_seed(State<double> *lam, const int32_t *inv, const int32_t *cells, int n, const double *seed,
+                             hipStream_t s);
+hipError_t launch_cells_map(const double *in, const int32_t *orig, double *out, int C, int to_external, hipStream_t s);
+hipError_t launch_states_map(State<double> *st, const int32_t *orig, double *H, double *U, double *V, double *Z, int C,
+                             int to_external, hipStream_t s);
 
 }  // namespace hydro

@@ -70,8 +70,20 @@ template <class R> struct Consts {
     static constexpr R BRDTH = (R)100.0;
 };
 
-template <class R> __host__ __device__ __forceinline__ R sqrt_r(double x) { return sqrtf((float)x); }
-template <> __host__ __device__ __forceinline__ double sqrt_r<double>(double x) { return sqrt(x); }
+// sqrt of the fp64 tree / sqrtf of the fp32 tree (argument rounded to float
+// first, as sqrtf(double) does); for the dual number, its own sqrt.
+template <class R> struct SqrtR {
+    template <class X> __host__ __device__ static __forceinline__ R f(X x) { return sqrt(x); }
+};
+template <> struct SqrtR<float> {
+    __host__ __device__ static __forceinline__ float f(double x) { return sqrtf((float)x); }
+};
+template <> struct SqrtR<double> {
+    __host__ __device__ static __forceinline__ double f(double x) { return sqrt(x); }
+};
+template <class R, class X> __host__ __device__ __forceinline__ R sqrt_r(X x) { return SqrtR<R>::f(x); }
+// pow evaluated in double (a float argument is promoted, as in the fp32 tree);
+// the dual number's overload is found by argument-dependent lookup.
 __host__ __device__ __forceinline__ double pow_d(double x, double y) { return pow(x, y); }
 
 template <class R> struct Flux { R f0, f1, f2, f3; };
@@ -81,9 +93,16 @@ template <class R> struct Scalars {
     R DT, HM1, HM2, JL;
 };
 
+// Element type of the arrays a Real of the arithmetic type R is read from
+// (the dual number of the tangent-linear model reads double arrays).
+template <class R> struct StorageOf {
+    using type = R;
+};
+
 // Everything the boundary handlers may read besides the cell states.
 template <class R> struct BcCtx {
-    const R *QT, *DQT, *ZT, *DZT, *ZW, *QW, *BF;
+    using T = typename StorageOf<R>::type;
+    const T *QT, *DQT, *ZT, *DZT, *ZW, *QW, *BF;
     const int32_t *NHQ1;
     int CELL, NHQ;
     int cell;   // ORIGINAL cell id (rows of the dense boundary arrays)
@@ -157,10 +176,10 @@ template <class R> __host__ __device__ inline R qd(R ZL, R ZR, R ZB) {
     return copysign(FI * HS * 0.3 * DH / 0.1, (double)(ZL - ZR));
 }
 
-template <class R> __host__ __device__ inline R laqp(R X, const R *A, const R *B, int MS) {
+template <class R, class T> __host__ __device__ inline R laqp(R X, const T *A, const T *B, int MS) {
     if (MS <= 0) return 0;
-    if (X < A[0]) return B[0];
-    if (X > A[MS - 1]) return B[MS - 1];
+    if (X < A[0]) return (R)B[0];
+    if (X > A[MS - 1]) return (R)B[MS - 1];
     for (int i = 0; i < MS - 1; i++)
         if (X >= A[i] && X <= A[i + 1]) return B[i] + (B[i + 1] - B[i]) / (A[i + 1] - A[i]) * (X - A[i]);
     return 0;
@@ -234,9 +253,9 @@ __host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, 
     case K_B3: {
         R QZH3 = (U1 * bc.COSF + V1 * bc.SINF) * H1 * bc.SIDE;
         QZH3 = fmax(QZH3, (R)0.0);
-        const R *QWr = bc.QW + (size_t)bc.cell * (size_t)bc.NHQ;
-        const R *ZWr = bc.ZW + (size_t)bc.cell * (size_t)bc.NHQ;
-        R ZQH1 = laqp<R>(QZH3, QWr, ZWr, bc.NHQ1[bc.cell]);
+        const auto *QWr = bc.QW + (size_t)bc.cell * (size_t)bc.NHQ;
+        const auto *ZWr = bc.ZW + (size_t)bc.cell * (size_t)bc.NHQ;
+        R ZQH1 = laqp(QZH3, QWr, ZWr, bc.NHQ1[bc.cell]);
         R ZQH11 = ZQH1 - Z1;
         R HB1 = ZQH1 - ZBp;
         HB1 = fmax(HB1, S.HM2);
@@ -263,7 +282,7 @@ __host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, 
         f.f3 = 4.905 * H1 * H1 * (1.0 - S.JL) * (1.0 - S.JL);
     } break;
     case K_B6: {
-        const R TOP = bc.BF[bc.cell];
+        const R TOP = (R)bc.BF[bc.cell];
         const R cs = bc.COSF, sn = bc.SINF;
         if (Z1 <= TOP && ZC <= TOP) {
             f.f0 = 0; f.f1 = 0; f.f2 = 0;
@@ -304,7 +323,7 @@ __host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, 
         }
     } break;
     case K_B7: {
-        const R TOP = bc.BF[bc.cell];
+        const R TOP = (R)bc.BF[bc.cell];
         const R HB = 0;   // the reference reads an uninitialised HB here
         if (Z1 > TOP || ZC > TOP) {
             *klas7_fired = true;

@@ -102,15 +102,41 @@ SIGNATURES = {
 _lib = None
 
 
+def _hip_runtimes_mapped():
+    """Paths of every libamdhip64 mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64" in line:
+                    paths.add(os.path.realpath(line.split()[-1]))
+    except OSError:
+        pass
+    return paths
+
+
 def load() -> ctypes.CDLL:
-    """Load libhydro_amd.so once; raise if it was not built."""
+    """Load libhydro_amd.so once; raise if it was not built.
+
+    PyTorch-ROCm ships its own libamdhip64 and requests it by a different
+    name than the library's DT_NEEDED, so whichever loads first decides
+    whether the process ends up with one HIP runtime or two.  Two runtimes
+    do not share allocations (our pointer queries would reject torch's
+    device memory), so torch is imported first and its runtime serves both.
+    """
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `make -C hydro-bench_amd` or __graft_entry__.build()")
+    import torch  # noqa: F401  (loads torch's HIP runtime first)
+
     lib = ctypes.CDLL(LIB_PATH)
+    runtimes = _hip_runtimes_mapped()
+    if len(runtimes) > 1:
+        raise ImportError(f"two HIP runtimes are loaded ({sorted(runtimes)}); import torch before "
+                          f"loading libhydro_amd.so")
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

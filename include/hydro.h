@@ -185,6 +185,47 @@ int hydro_dump(hydro_mesh *m, const char *path, hipStream_t stream);
 int hydro_set_profiling(hydro_mesh *m, int enable);
 int hydro_last_timing(const hydro_mesh *m, double *total_ms, int *launches, int *steps);
 
+/* ---- adjoint and tangent-linear steps (fp64, variant A) ------------------- */
+/* No counterpart in the reference: these make gradients of a gauge misfit
+ * with respect to per-cell Manning n (through FNC = 9.81 n^2) and bed ZBC
+ * available.  They work on the reordered layout; every buffer is caller-owned
+ * device memory of the size hydro_adj_sizes reports, in the library's
+ * internal cell order:
+ *   state   {H, U, V, Z} per cell       (state_bytes)
+ *   wsum    one step's flux sums [3][C] (wsum_bytes)
+ *   cellvec one value per cell          (cellvec_bytes)
+ * A step's time (kt, jt) is passed explicitly (the step that produced the
+ * state after it). */
+int hydro_adj_sizes(const hydro_mesh *m, size_t *state_bytes, size_t *wsum_bytes, size_t *cellvec_bytes);
+/* current state <-> a state buffer (checkpoints) */
+int hydro_state_save(hydro_mesh *m, void *state, hipStream_t stream);
+int hydro_state_load(hydro_mesh *m, const void *state, hipStream_t stream);
+/* nsteps fused steps from the current state and time; states + k*state_bytes
+ * receives the input state of step k, wsums + k*wsum_bytes its flux sums */
+int hydro_step_record(hydro_mesh *m, int nsteps, void *states, void *wsums, hipStream_t stream);
+/* lam_in = (d state_out / d state_in)^T lam_out for the step (kt, jt) whose
+ * input state and flux sums were recorded; g_fnc, g_zbc += parameter parts.
+ * Slot kinds without an adjoint (KLAS 3, 5, 6, 7, unknown codes) raise a
+ * flag read by hydro_adjoint_check. */
+int hydro_adjoint_step(hydro_mesh *m, int kt, int jt, const void *s_in, const void *wsum, const void *lam_out,
+                       void *lam_in, void *g_fnc, void *g_zbc, hipStream_t stream);
+int hydro_adjoint_check(hydro_mesh *m, int *unsupported);
+/* tangent-linear step: s_out = step(s_in), ds_out = d step . (ds_in, dfnc, dzbc) */
+int hydro_tlm_step(hydro_mesh *m, int kt, int jt, const void *s_in, const void *ds_in, const void *dfnc,
+                   const void *dzbc, void *s_out, void *ds_out, hipStream_t stream);
+/* gauges are ORIGINAL cell ids (device int32[n]): read Z of a state buffer,
+ * or add seed[i] to the Z component of an adjoint buffer */
+int hydro_gauge_values(hydro_mesh *m, const void *state, const int32_t *cells, int n, double *out,
+                       hipStream_t stream);
+int hydro_gauge_seed(hydro_mesh *m, void *lam, const int32_t *cells, int n, const double *seed, hipStream_t stream);
+/* per-cell vectors and state buffers between internal and caller order */
+int hydro_cells_map(hydro_mesh *m, const double *in, double *out, int to_external, hipStream_t stream);
+int hydro_states_map(hydro_mesh *m, void *state, double *H, double *U, double *V, double *Z, int to_external,
+                     hipStream_t stream);
+/* re-read ZBC, AREA, FNC (and ZB1 when stored) from the caller's arrays after
+ * the caller changed them (e.g. a calibration update) */
+int hydro_refresh_statics(hydro_mesh *m, hipStream_t stream);
+
 /* ---- host-side mesh preparation ------------------------------------------ */
 
 /* Geometry preprocessing of MeshData::preCalculate
