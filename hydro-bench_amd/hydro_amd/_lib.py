@@ -97,6 +97,20 @@ SIGNATURES = {
     "hydro_precalculate": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_double] + [c_void_p] * 9),
     "hydro_rcm_order": (c_int, [c_int, c_void_p, c_void_p]),
+    # adjoint / tangent-linear support
+    "hydro_adj_sizes": (c_int, [c_void_p, POINTER(ctypes.c_size_t), POINTER(ctypes.c_size_t),
+                                POINTER(ctypes.c_size_t)]),
+    "hydro_state_save": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "hydro_state_load": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "hydro_step_record": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "hydro_adjoint_step": (c_int, [c_void_p, c_int, c_int] + [c_void_p] * 7),
+    "hydro_adjoint_check": (c_int, [c_void_p, POINTER(c_int)]),
+    "hydro_tlm_step": (c_int, [c_void_p, c_int, c_int] + [c_void_p] * 7),
+    "hydro_gauge_values": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "hydro_gauge_seed": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "hydro_cells_map": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "hydro_states_map": (c_int, [c_void_p] * 6 + [c_int, c_void_p]),
+    "hydro_refresh_statics": (c_int, [c_void_p, c_void_p]),
 }
 
 _lib = None
