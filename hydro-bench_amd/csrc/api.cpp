@@ -173,6 +173,7 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.SLSIN = m->need_slcs ? m->SLSIN.as<R>() : nullptr;
     f.orig = m->orig.as<int32_t>();
     f.materialize = m->tune.materialize_flux;
+    f.occ4 = m->tune.waves_per_simd == 4;
     f.rec_state = nullptr;
     f.rec_wsum = nullptr;
     return f;
@@ -469,6 +470,7 @@ hydro_tune default_tune() {
     t.graph_chunk = 16;
     t.persistent_wgs_per_cu = 0;
     t.materialize_flux = 0;
+    t.waves_per_simd = 0;
     return t;
 }
 
@@ -554,6 +556,8 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     if (t.block < 64 || t.block > kFusedMaxBlock || (t.block % 64))
         return fail(HYDRO_ERR_INVALID, "block %d (64..%d, multiple of 64)", t.block, kFusedMaxBlock);
     if (t.reorder < 0 || t.reorder > 2) return fail(HYDRO_ERR_INVALID, "reorder %d", t.reorder);
+    if (t.waves_per_simd != 0 && t.waves_per_simd != 4)
+        return fail(HYDRO_ERR_INVALID, "waves_per_simd %d (0 or 4)", t.waves_per_simd);
     // the internal state must be written back before the layout is replaced
     int rc = HYDRO_OK;
     if (m->prepared && !m->ext_valid)
@@ -873,6 +877,198 @@ int hydro_refresh_statics(hydro_mesh *m, hipStream_t s) {
         HIP_TRY(launch_refresh_statics<float>(ext_view<float>(m), m->orig.as<int32_t>(), m->ZBC.as<float>(),
                                               m->AREA.as<float>(), m->FNC.as<float>(),
                                               m->need_zb1 ? m->ZB1.as<float>() : nullptr, m->CELL, s));
+    }
+    return HYDRO_OK;
+}
+
+// ---- batches ------------------------------------------------------------------
+
+}  // extern "C"
+
+struct hydro_batch {
+    std::vector<hydro_mesh *> meshes;
+    int dtype = 0, variant = 0, block = 256, remap = 1, chunk = 16;
+    int ntiles = 0;
+    DevBuf views, tile_mesh, tile0, g0, spd, tick, barrier, perr;
+    std::vector<int32_t> h_tile0;
+    hipStream_t cap_stream = nullptr;
+    std::map<GraphKey, std::unique_ptr<GraphExec>> graphs;
+    ~hydro_batch() {
+        graphs.clear();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+};
+
+namespace {
+
+template <class R> int batch_upload(hydro_batch *b, hipStream_t s, std::vector<char> &host_views) {
+    const int n = (int)b->meshes.size();
+    host_views.resize(sizeof(FusedView<R>) * n);
+    std::vector<int64_t> g0(n);
+    std::vector<int32_t> spd(n);
+    for (int i = 0; i < n; i++) {
+        hydro_mesh *m = b->meshes[i];
+        int rc = ensure_internal<R>(m, s);
+        if (rc) return rc;
+        FusedView<R> f = fused_view<R>(m, m->cur);
+        std::memcpy(host_views.data() + sizeof(FusedView<R>) * i, &f, sizeof f);
+        g0[i] = global_step(m);
+        spd[i] = m->spd;
+    }
+    HIP_TRY(hipMemcpyAsync(b->views.p, host_views.data(), host_views.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->g0.p, g0.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->spd.p, spd.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));   // host_views are pageable and short-lived
+    return HYDRO_OK;
+}
+
+template <class R> BatchView<R> batch_view(const hydro_batch *b) {
+    BatchView<R> v;
+    v.views = b->views.as<FusedView<R>>();
+    v.tile_mesh = b->tile_mesh.as<int32_t>();
+    v.tile0 = b->tile0.as<int32_t>();
+    v.g0 = b->g0.as<int64_t>();
+    v.spd = b->spd.as<int32_t>();
+    v.ntiles = b->ntiles;
+    return v;
+}
+
+void batch_advance(hydro_batch *b, int n) {
+    for (hydro_mesh *m : b->meshes) {
+        for (int i = 0; i < n; i++) advance(m);
+        if (n & 1) m->cur ^= 1;
+        m->ext_valid = false;
+    }
+}
+
+template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hipStream_t s) {
+    std::vector<char> hv;
+    int rc = batch_upload<R>(b, s, hv);
+    if (rc) return rc;
+    const BatchView<R> v = batch_view<R>(b);
+    if (mode == HYDRO_MODE_FUSED) {
+        for (int i = 0; i < n; i++) HIP_TRY((launch_batch_step<R, VAR>(v, i, b->block, b->remap, s)));
+    } else if (mode == HYDRO_MODE_GRAPH) {
+        int chunk = std::max(2, b->chunk);
+        if (chunk & 1) chunk++;
+        if (!b->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        int done = 0;
+        if (n >= chunk) {
+            GraphKey key{HYDRO_MODE_GRAPH, 0, chunk};
+            auto it = b->graphs.find(key);
+            if (it == b->graphs.end()) {
+                auto ge = std::make_unique<GraphExec>();
+                HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+                hipError_t err = hipSuccess;
+                for (int k = 0; k < chunk && err == hipSuccess; k++)
+                    err = launch_batch_step_tick<R, VAR>(v, b->tick.as<int64_t>(), k, b->block, b->remap, b->cap_stream);
+                if (err == hipSuccess) err = launch_tick_advance(b->tick.as<int64_t>(), chunk, b->cap_stream);
+                hipError_t e2 = hipStreamEndCapture(b->cap_stream, &ge->g);
+                if (err != hipSuccess || e2 != hipSuccess)
+                    return fail(HYDRO_ERR_HIP, "batch graph capture failed: %s", hipGetErrorString(err ? err : e2));
+                HIP_TRY(hipGraphInstantiate(&ge->x, ge->g, nullptr, nullptr, 0));
+                it = b->graphs.emplace(key, std::move(ge)).first;
+            }
+            hipLaunchKernelGGL(k_set_tick, dim3(1), dim3(64), 0, s, b->tick.as<int64_t>(), (int64_t)0);
+            HIP_TRY(hipGetLastError());
+            while (n - done >= chunk) {
+                HIP_TRY(hipGraphLaunch(it->second->x, s));
+                done += chunk;
+            }
+        }
+        for (int i = done; i < n; i++) HIP_TRY((launch_batch_step<R, VAR>(v, i, b->block, b->remap, s)));
+    } else if (mode == HYDRO_MODE_PERSISTENT) {
+        if (n > 0) {
+            int maxg = 0;
+            HIP_TRY((batch_persistent_max_grid<R, VAR>(b->block, &maxg)));
+            const int grid = std::max(1, std::min(maxg, b->ntiles));
+            PersistentArgs pa;
+            pa.nsteps = n;
+            pa.day0 = pa.kt0 = pa.spd = 0;   // per-mesh time comes from the batch tables
+            pa.barrier = b->barrier.as<unsigned>();
+            pa.error = b->perr.as<int>();
+            pa.spin_limit = 1u << 24;
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY((launch_batch_persistent<R, VAR>(v, pa, b->block, grid, s)));
+        }
+    } else {
+        return fail(HYDRO_ERR_UNSUPPORTED, "batch mode %d (FUSED, GRAPH or PERSISTENT)", mode);
+    }
+    batch_advance(b, n);
+    return HYDRO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune, hydro_batch **out) {
+    if (!meshes || n <= 0 || !out) return fail(HYDRO_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    hydro_tune t = tune ? *tune : default_tune();
+    auto b = std::make_unique<hydro_batch>();
+    b->dtype = meshes[0]->d.dtype;
+    b->variant = meshes[0]->d.variant;
+    b->block = t.block;
+    b->remap = t.xcd_remap;
+    b->chunk = t.graph_chunk;
+    std::vector<int32_t> tile_mesh, tile0(n);
+    for (int i = 0; i < n; i++) {
+        hydro_mesh *m = meshes[i];
+        if (!m) return fail(HYDRO_ERR_INVALID, "mesh %d is NULL", i);
+        if (m->d.dtype != b->dtype || m->d.variant != b->variant)
+            return fail(HYDRO_ERR_INVALID, "mesh %d: every mesh of a batch needs the same dtype and variant", i);
+        for (int j = 0; j < i; j++)
+            if (meshes[j] == m) return fail(HYDRO_ERR_INVALID, "mesh %d appears twice", i);
+        if (!m->prepared || m->tune.block != t.block || m->tune.reorder != t.reorder ||
+            m->tune.materialize_flux != t.materialize_flux) {
+            int rc = hydro_prepare(m, &t);
+            if (rc) return rc;
+        }
+        tile0[i] = (int32_t)tile_mesh.size();
+        const int nt = (m->CELL + t.block - 1) / t.block;
+        for (int k = 0; k < nt; k++) tile_mesh.push_back(i);
+        b->meshes.push_back(m);
+    }
+    b->ntiles = (int)tile_mesh.size();
+    b->h_tile0 = tile0;
+    const size_t vsz = (b->dtype == HYDRO_F64) ? sizeof(FusedView<double>) : sizeof(FusedView<float>);
+    int rc;
+    if ((rc = b->views.alloc(vsz * n)) || (rc = h2d(b->tile_mesh, tile_mesh)) || (rc = h2d(b->tile0, tile0)) ||
+        (rc = b->g0.alloc(sizeof(int64_t) * n)) || (rc = b->spd.alloc(sizeof(int32_t) * n)) ||
+        (rc = b->tick.alloc(sizeof(int64_t))) || (rc = b->barrier.alloc(sizeof(unsigned))) ||
+        (rc = b->perr.alloc(sizeof(int))))
+        return rc;
+    HIP_TRY(hipMemset(b->perr.p, 0, sizeof(int)));
+    *out = b.release();
+    return HYDRO_OK;
+}
+
+int hydro_batch_destroy(hydro_batch *b) {
+    delete b;
+    return HYDRO_OK;
+}
+
+int hydro_batch_step_n(hydro_batch *b, int nsteps, int mode, hipStream_t stream) {
+    if (!b || nsteps < 0) return fail(HYDRO_ERR_INVALID, "bad arguments");
+    for (size_t i = 0; i < b->meshes.size(); i++) {
+        hydro_mesh *m = b->meshes[i];
+        const int per = m->spd - 1;
+        const int64_t rem = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
+        if (nsteps > rem)
+            return fail(HYDRO_ERR_RANGE, "mesh %zu: %d steps requested, %lld remain", i, nsteps, (long long)rem);
+    }
+    if (b->dtype == HYDRO_F64)
+        return b->variant ? batch_steps<double, 1>(b, nsteps, mode, stream) : batch_steps<double, 0>(b, nsteps, mode, stream);
+    return b->variant ? batch_steps<float, 1>(b, nsteps, mode, stream) : batch_steps<float, 0>(b, nsteps, mode, stream);
+}
+
+int hydro_batch_sync(hydro_batch *b, hipStream_t stream) {
+    if (!b) return fail(HYDRO_ERR_INVALID, "null batch");
+    for (hydro_mesh *m : b->meshes) {
+        int rc = hydro_sync(m, stream);
+        if (rc) return rc;
     }
     return HYDRO_OK;
 }

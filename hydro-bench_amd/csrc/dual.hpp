@@ -85,6 +85,7 @@ __host__ __device__ inline Dual pow_d(Dual x, double y) {
     const double p = ::pow(x.v, y);
     return Dual(p, x.v != 0 ? y * p / x.v * x.d : 0.0);
 }
+__host__ __device__ inline Dual pow_rare(Dual x, double y) { return pow_d(x, y); }
 
 }  // namespace ad
 }  // namespace hydro

@@ -185,6 +185,16 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, i
     fused_cell<R, VAR>(f, f.Sin, f.Sout, p, kt, jt);
 }
 
+// Same step limited to 128 VGPRs so four waves fit on each SIMD.
+template <class R, int VAR>
+__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_fused_step_o4(FusedView<R> f, int kt, int jt, int remap) {
+    const int t = tile_of(blockIdx.x, gridDim.x, remap);
+    const int p = t * blockDim.x + threadIdx.x;
+    if (p >= f.C) return;
+    fused_cell<R, VAR>(f, f.Sin, f.Sout, p, kt, jt);
+}
+
 // (day, kt) of global step g counted from day 0, kt 1 (benchmark.cu:89-90).
 __device__ __forceinline__ void time_of(int64_t g, int spd, int *day, int *kt) {
     const int per = spd - 1;
@@ -198,6 +208,17 @@ __device__ __forceinline__ void time_of(int64_t g, int spd, int *day, int *kt) {
 template <class R, int VAR>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_tick(FusedView<R> f, const int64_t *tick,
                                                                     int offset, int spd, int remap) {
+    const int t = tile_of(blockIdx.x, gridDim.x, remap);
+    const int p = t * blockDim.x + threadIdx.x;
+    if (p >= f.C) return;
+    int day, kt;
+    time_of(*tick + offset, spd, &day, &kt);
+    fused_cell<R, VAR>(f, f.Sin, f.Sout, p, kt, day);
+}
+
+template <class R, int VAR>
+__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_fused_step_tick_o4(FusedView<R> f, const int64_t *tick, int offset, int spd, int remap) {
     const int t = tile_of(blockIdx.x, gridDim.x, remap);
     const int p = t * blockDim.x + threadIdx.x;
     if (p >= f.C) return;
@@ -220,10 +241,12 @@ __global__ void k_tick_advance(int64_t *tick, int by) {
 // a broken launch cannot hang the device.
 __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target, int *error,
                                              unsigned spin_limit) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wave waits until its own stores are complete (written through to
+    // this XCD's L2), so one device-scope release per workgroup covers them
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
         while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
@@ -232,9 +255,9 @@ __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target,
                 break;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 template <class R, int VAR>
@@ -253,6 +276,43 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_persistent(FusedView<R> f, P
         State<R> *tmp = const_cast<State<R> *>(a);
         a = b;
         b = tmp;
+        if (s + 1 < pa.nsteps) {
+            grid_barrier(pa.barrier, (unsigned)(s + 1) * gridDim.x, pa.error, pa.spin_limit);
+            if (__hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batches of meshes.
+
+template <class R, int VAR>
+__device__ __forceinline__ void batch_tile(const BatchView<R> &b, int t, int step) {
+    const int m = __builtin_amdgcn_readfirstlane(b.tile_mesh[t]);
+    const FusedView<R> &f = b.views[m];
+    const int p = (t - b.tile0[m]) * blockDim.x + threadIdx.x;
+    if (p >= f.C) return;
+    int day, kt;
+    time_of(b.g0[m] + step, b.spd[m], &day, &kt);
+    const bool odd = step & 1;
+    fused_cell<R, VAR>(f, odd ? f.Sout : f.Sin, odd ? const_cast<State<R> *>(f.Sin) : f.Sout, p, kt, day);
+}
+
+template <class R, int VAR>
+__global__ void __launch_bounds__(kFusedMaxBlock) k_batch_step(BatchView<R> b, int step, int remap) {
+    batch_tile<R, VAR>(b, tile_of(blockIdx.x, gridDim.x, remap), step);
+}
+
+template <class R, int VAR>
+__global__ void __launch_bounds__(kFusedMaxBlock) k_batch_step_tick(BatchView<R> b, const int64_t *tick, int offset,
+                                                                    int remap) {
+    batch_tile<R, VAR>(b, tile_of(blockIdx.x, gridDim.x, remap), (int)(*tick) + offset);
+}
+
+template <class R, int VAR>
+__global__ void __launch_bounds__(kFusedMaxBlock) k_batch_persistent(BatchView<R> b, PersistentArgs pa) {
+    for (int s = 0; s < pa.nsteps; s++) {
+        for (int t = blockIdx.x; t < b.ntiles; t += gridDim.x) batch_tile<R, VAR>(b, t, s);
         if (s + 1 < pa.nsteps) {
             grid_barrier(pa.barrier, (unsigned)(s + 1) * gridDim.x, pa.error, pa.spin_limit);
             if (__hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -343,7 +403,10 @@ template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_fused_step<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, kt, jt, remap);
+    if (f.occ4)
+        hipLaunchKernelGGL((k_fused_step_o4<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, kt, jt, remap);
+    else
+        hipLaunchKernelGGL((k_fused_step<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, kt, jt, remap);
     return hipGetLastError();
 }
 
@@ -352,8 +415,12 @@ hipError_t launch_fused_step_tick(const FusedView<R> &f, const int64_t *tick, in
                                   int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_fused_step_tick<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, tick,
-                       offset, spd, remap);
+    if (f.occ4)
+        hipLaunchKernelGGL((k_fused_step_tick_o4<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, tick,
+                           offset, spd, remap);
+    else
+        hipLaunchKernelGGL((k_fused_step_tick<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, tick,
+                           offset, spd, remap);
     return hipGetLastError();
 }
 
@@ -406,6 +473,46 @@ hipError_t launch_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uin
     return hipGetLastError();
 }
 
+template <class R, int VAR>
+hipError_t launch_batch_step(const BatchView<R> &b, int step, int block, int remap, hipStream_t s) {
+    if (b.ntiles == 0) return hipSuccess;
+    if (block > kFusedMaxBlock) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_batch_step<R, VAR>), dim3(b.ntiles), dim3(block), 0, s, b, step, remap);
+    return hipGetLastError();
+}
+
+template <class R, int VAR>
+hipError_t launch_batch_step_tick(const BatchView<R> &b, const int64_t *tick, int offset, int block, int remap,
+                                  hipStream_t s) {
+    if (b.ntiles == 0) return hipSuccess;
+    if (block > kFusedMaxBlock) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_batch_step_tick<R, VAR>), dim3(b.ntiles), dim3(block), 0, s, b, tick, offset, remap);
+    return hipGetLastError();
+}
+
+template <class R, int VAR>
+hipError_t batch_persistent_max_grid(int block, int *grid) {
+    int dev = 0, per_cu = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_batch_persistent<R, VAR>, block, 0);
+    if (e != hipSuccess) return e;
+    *grid = per_cu * cus;
+    return hipSuccess;
+}
+
+template <class R, int VAR>
+hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &pa, int block, int grid,
+                                   hipStream_t s) {
+    if (block > kFusedMaxBlock) return hipErrorInvalidValue;
+    BatchView<R> bb = b;
+    PersistentArgs p = pa;
+    void *args[] = {&bb, &p};
+    return hipLaunchCooperativeKernel((const void *)k_batch_persistent<R, VAR>, dim3(grid), dim3(block), args, 0, s);
+}
+
 #define HYDRO_INSTANTIATE(R, VAR)                                                                        \
     template hipError_t launch_calculate_flux<R, VAR>(const ExtView<R> &, int, int, int, hipStream_t);   \
     template hipError_t launch_update_cell<R, VAR>(const ExtView<R> &, int, hipStream_t);               \
@@ -414,7 +521,13 @@ hipError_t launch_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uin
                                                        int, int, hipStream_t);                          \
     template hipError_t launch_persistent<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, int, \
                                                   hipStream_t);                                         \
-    template hipError_t persistent_max_grid<R, VAR>(int, int *);
+    template hipError_t persistent_max_grid<R, VAR>(int, int *);                                       \
+    template hipError_t launch_batch_step<R, VAR>(const BatchView<R> &, int, int, int, hipStream_t);   \
+    template hipError_t launch_batch_step_tick<R, VAR>(const BatchView<R> &, const int64_t *, int, int, int, \
+                                                       hipStream_t);                                    \
+    template hipError_t launch_batch_persistent<R, VAR>(const BatchView<R> &, const PersistentArgs &, int, int, \
+                                                        hipStream_t);                                   \
+    template hipError_t batch_persistent_max_grid<R, VAR>(int, int *);
 
 HYDRO_INSTANTIATE(double, 0)
 HYDRO_INSTANTIATE(double, 1)

@@ -29,6 +29,7 @@ template <class R> struct alignas(4 * sizeof(R)) State {
 template <class R> struct FusedView {
     int C;
     int materialize;           // also write the caller's FLUX0..3 every step
+    int occ4;                  // launch the 128-VGPR variant (4 waves per SIMD)
     Scalars<R> S;
     // the caller's arrays, read only by boundary slots, KLAS 7 rewrites and
     // materialised fluxes; kept behind a device pointer so the common path
@@ -90,6 +91,30 @@ hipError_t launch_scatter(const ExtView<R> &m, const int32_t *orig, const State<
                           int C, hipStream_t s);
 hipError_t launch_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uint8_t *meta, int C,
                                hipStream_t s);
+// ---- batches: many meshes advanced by one launch per step --------------------
+// Each workgroup tile belongs to exactly one mesh (tile_mesh[t]); the mesh's
+// tiles are [tile0[m], tile0[m] + ceil(C_m / block)).  Every mesh keeps its own
+// reordered buffers; views[m] points at them (Sin = state before the batch's
+// first step).  Mesh m's step s runs at global time g0[m] + s.
+template <class R> struct BatchView {
+    const FusedView<R> *views;
+    const int32_t *tile_mesh;
+    const int32_t *tile0;
+    const int64_t *g0;
+    const int32_t *spd;
+    int ntiles;
+};
+template <class R, int VAR>
+hipError_t launch_batch_step(const BatchView<R> &b, int step, int block, int remap, hipStream_t s);
+template <class R, int VAR>
+hipError_t launch_batch_step_tick(const BatchView<R> &b, const int64_t *tick, int offset, int block, int remap,
+                                  hipStream_t s);
+template <class R, int VAR>
+hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &pa, int block, int grid,
+                                   hipStream_t s);
+template <class R, int VAR>
+hipError_t batch_persistent_max_grid(int block, int *grid);
+
 // internal statics <- the caller's ZBC AREA FNC (and ZB1 when stored)
 template <class R>
 hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *ZB1, int C,

@@ -85,6 +85,9 @@ template <class R, class X> __host__ __device__ __forceinline__ R sqrt_r(X x) { 
 // pow evaluated in double (a float argument is promoted, as in the fp32 tree);
 // the dual number's overload is found by argument-dependent lookup.
 __host__ __device__ __forceinline__ double pow_d(double x, double y) { return pow(x, y); }
+// The same pow, kept out of line: used on the rare branches (wet/dry edges,
+// boundary handlers) so their expansions do not bloat the common path.
+__host__ __device__ __attribute__((noinline)) inline double pow_rare(double x, double y) { return pow(x, y); }
 
 template <class R> struct Flux { R f0, f1, f2, f3; };
 
@@ -170,7 +173,7 @@ template <class R> __host__ __device__ inline R qd(R ZL, R ZR, R ZB) {
     R ZU = fmax(ZL, ZR), ZD = fmin(ZL, ZR);
     R H0 = ZU - ZB, HS = ZD - ZB;
     R DELTA = HS / H0;
-    if (DELTA <= SIGMA) return copysign(CM * pow_d(H0, 1.5), (double)(ZL - ZR));
+    if (DELTA <= SIGMA) return copysign(CM * pow_rare(H0, 1.5), (double)(ZL - ZR));
     R DH = ZU - ZD;
     if (DH > 0.09) return copysign(FI * HS * sqrt_r<R>(DH), ZL - ZR);
     return copysign(FI * HS * 0.3 * DH / 0.1, (double)(ZL - ZR));
@@ -203,7 +206,7 @@ template <class R> __host__ __device__ inline R urb_iterate(R QLn, R H_pre, R HB
 // that do not write them.  Sets *klas7_fired when a KLAS 7 slot turns into
 // an interior slot (F:242).
 template <class R, int VAR>
-__host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
+__host__ __device__ __attribute__((noinline)) Flux<R> boundary_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
                                                  R H1, R U1, R V1, R Z1, R ZBp, R ZC, R UC, R VC,
                                                  R QL1, R QL2, R FIL, Flux<R> prev,
                                                  bool *klas7_fired) {
@@ -288,15 +291,15 @@ __host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, 
             f.f0 = 0; f.f1 = 0; f.f2 = 0;
             f.f3 = 4.905 * H1 * H1;
         } else if (Z1 > TOP && ZC <= TOP) {
-            f.f0 = K::C0 * pow_d(Z1 - TOP, 1.5);
+            f.f0 = K::C0 * pow_rare(Z1 - TOP, 1.5);
             f.f1 = f.f0 * QL1;
             f.f2 = f.f0 * QL2;
-            f.f3 = 4.905 * pow_d(TOP - ZBp, 2.0);
+            f.f3 = 4.905 * pow_rare(TOP - ZBp, 2.0);
         } else if (Z1 <= TOP && ZC > TOP) {
-            f.f0 = -K::C0 * pow_d(ZC - TOP, 1.5);
+            f.f0 = -K::C0 * pow_rare(ZC - TOP, 1.5);
             f.f1 = f.f0 * fmin(UC * cs + VC * sn, (R)0.0);
             f.f2 = f.f0 * (VC * cs - UC * sn);
-            f.f3 = 4.905 * pow_d(Z1 - ZBp, 2.0);
+            f.f3 = 4.905 * pow_rare(Z1 - ZBp, 2.0);
         } else {
             R DZ = fabs(Z1 - ZC);
             if (Z1 <= ZC) {
@@ -304,9 +307,9 @@ __host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, 
                 R UN = fmin(UC * cs + VC * sn, (R)0.0);
                 R VT = VC * cs - UC * sn;
                 R SH = HD + DZ;
-                R CE = fmin(1.0, 1.05 * pow_d(DZ / SH, 0.33333));
+                R CE = fmin(1.0, 1.05 * pow_rare(DZ / SH, 0.33333));
                 if (Z1 < ZC && UN > 0.0) UN = 0.0;
-                f.f0 = copysign(CE * K::C1 * pow_d(SH, 1.5), (double)(Z1 - ZC));
+                f.f0 = copysign(CE * K::C1 * pow_rare(SH, 1.5), (double)(Z1 - ZC));
                 f.f1 = f.f0 * fabs(UN);
                 f.f2 = f.f0 * VT;
             } else {
@@ -314,12 +317,12 @@ __host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, 
                 R UN = fmax(QL1, (R)0.0);
                 R VT = QL2;
                 R SH = HD + DZ;
-                R CE = fmin(1.0, 1.05 * pow_d(DZ / SH, 0.33333));
-                f.f0 = copysign(CE * K::C1 * pow_d(SH, 1.5), (double)(Z1 - ZC));
+                R CE = fmin(1.0, 1.05 * pow_rare(DZ / SH, 0.33333));
+                f.f0 = copysign(CE * K::C1 * pow_rare(SH, 1.5), (double)(Z1 - ZC));
                 f.f1 = (Z1 - ZC) * fabs(UN) * UN;
                 f.f2 = (Z1 - ZC) * fabs(UN) * VT;
             }
-            f.f3 = 4.905 * pow_d(TOP - ZBp, 2.0);
+            f.f3 = 4.905 * pow_rare(TOP - ZBp, 2.0);
         }
     } break;
     case K_B7: {
@@ -376,14 +379,14 @@ __host__ __device__ inline Flux<R> slot_flux(int kind, const Scalars<R> &S, cons
         return f;
     }
     if (ZI <= BC) {
-        f.f0 = -K::C1 * pow_d(HC, 1.5);
+        f.f0 = -K::C1 * pow_rare(HC, 1.5);
         f.f1 = H1 * QL1 * fabs(QL1);
         f.f2 = 0;
         f.f3 = 4.905 * H1 * H1;
         return f;
     }
     if (ZC <= ZBp) {
-        f.f0 = K::C1 * pow_d(H1, 1.5);
+        f.f0 = K::C1 * pow_rare(H1, 1.5);
         f.f1 = H1 * fabs(QL1) * QL1;
         f.f2 = H1 * fabs(QL1) * QL2;
         f.f3 = 0;
@@ -397,7 +400,7 @@ __host__ __device__ inline Flux<R> slot_flux(int kind, const Scalars<R> &S, cons
             f.f1 = f.f0 * UN;
             f.f2 = f.f0 * (VC * COSJ - UC * SINJ);
         } else {
-            f.f0 = K::C1 * pow_d(H1, 1.5);
+            f.f0 = K::C1 * pow_rare(H1, 1.5);
             f.f1 = 0;
             f.f2 = 0;
         }
@@ -414,7 +417,7 @@ __host__ __device__ inline Flux<R> slot_flux(int kind, const Scalars<R> &S, cons
             f.f2 = f.f0 * QL2;
             f.f3 = 4.905 * HC1 * HC1;
         } else {
-            f.f0 = -K::C1 * pow_d(HC, 1.5);
+            f.f0 = -K::C1 * pow_rare(HC, 1.5);
             f.f1 = H1 * QL1 * QL1;
             f.f2 = 0;
             f.f3 = 4.905 * H1 * H1;

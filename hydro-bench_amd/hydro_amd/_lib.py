@@ -65,13 +65,14 @@ class Tune(ctypes.Structure):
 
     _fields_ = [
         ("block", c_int32), ("reorder", c_int32), ("xcd_remap", c_int32), ("graph_chunk", c_int32),
-        ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("reserved", c_int32 * 2),
+        ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("waves_per_simd", c_int32),
+        ("reserved", c_int32),
     ]
 
 
 def default_tune(**kw) -> Tune:
     t = Tune(block=256, reorder=REORDER_RCM, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
-             materialize_flux=0)
+             materialize_flux=0, waves_per_simd=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t

@@ -127,10 +127,13 @@ class DeviceMesh:
 
     # -- state helpers ----------------------------------------------------------
     def load_state(self, state: dict):
-        """Overwrite H U V Z (and W) from host arrays, then re-gather."""
+        """Overwrite H U V Z (and W) from host arrays or tensors, then re-gather."""
         torch = _torch()
         for k, v in state.items():
-            self.t[k].copy_(torch.from_numpy(np.ascontiguousarray(v)).to(self.device))
+            if isinstance(v, torch.Tensor):
+                self.t[k].copy_(v)
+            else:
+                self.t[k].copy_(torch.from_numpy(np.ascontiguousarray(v)).to(self.device))
         self.reload()
 
     def fetch(self, *names):

@@ -125,12 +125,13 @@ def _delaunay(pts):
 def _merge_pairs(pts, tri, n_merge, rng):
     """Merge n_merge adjacent triangle pairs into convex quads, best shaped first.
 
-    Returns a list of cells, each a tuple of 3 or 4 CCW node ids, in the
-    order of the triangles (a quad takes the place of its first triangle).
+    Returns an int64 array [C, 4] of CCW node ids (-1 in the 4th column of a
+    triangle), in the order of the triangles (a quad takes the place of its
+    first triangle).
     """
     T = len(tri)
     if n_merge <= 0:
-        return [tuple(t) for t in tri.tolist()]
+        return np.concatenate([tri, np.full((T, 1), -1, dtype=np.int64)], axis=1)
     N = len(pts)
     # half-edges (t, k): edge from tri[t,k] to tri[t,(k+1)%3]
     va = tri.ravel()
@@ -160,40 +161,32 @@ def _merge_pairs(pts, tri, n_merge, rng):
     cand = np.nonzero(convex)[0]
     cand = cand[np.argsort(score[cand], kind="stable")]
     used = np.zeros(T, dtype=bool)
-    merged = {}
+    cells = np.concatenate([tri, np.full((T, 1), -1, dtype=np.int64)], axis=1)
+    drop = np.zeros(T, dtype=bool)
     count = 0
+    t1l, t2l = t1.tolist(), t2.tolist()
     for e in cand.tolist():
         if count >= n_merge:
             break
-        x, y = int(t1[e]), int(t2[e])
+        x, y = t1l[e], t2l[e]
         if used[x] or used[y]:
             continue
         used[x] = used[y] = True
-        lo, hi = min(x, y), max(x, y)
-        merged[lo] = tuple(int(v) for v in quad[e])
-        merged[hi] = None
+        lo, hi = (x, y) if x < y else (y, x)
+        cells[lo] = quad[e]
+        drop[hi] = True
         count += 1
     if count < n_merge:
         raise ValueError(f"only {count} of {n_merge} quad merges possible")
-    cells = []
-    for t in range(T):
-        if t in merged:
-            if merged[t] is not None:
-                cells.append(merged[t])
-        else:
-            cells.append(tuple(int(v) for v in tri[t]))
-    return cells
+    return cells[~drop]
 
 
 def _topology(cells, n_nodes):
     """NAP [4*C] (1-based, triangle 4th = 0), NAC [4*C] (1-based, 0 = none),
     and per-slot edge endpoints for boundary classification."""
     C = len(cells)
-    NAP = np.zeros((C, 4), dtype=np.int32)
-    nv = np.zeros(C, dtype=np.int32)
-    for i, cell in enumerate(cells):
-        NAP[i, : len(cell)] = np.asarray(cell) + 1
-        nv[i] = len(cell)
+    NAP = (cells + 1).astype(np.int32)                # -1 padding becomes 0
+    nv = np.where(cells[:, 3] >= 0, 4, 3).astype(np.int32)
     cid = np.repeat(np.arange(C), 4)
     j = np.tile(np.arange(4), C)
     valid = j < nv[cid]
@@ -307,8 +300,11 @@ def _raw_mesh(Lx, Ly, target_cells, quad_frac, rng, aspect=None):
     assert len(cells) == target_cells, (len(cells), target_cells)
     NAP, NAC, valid, n1, n2 = _topology(cells, len(pts))
     C = len(cells)
-    cx = np.array([pts[list(c), 0].mean() for c in cells])
-    cy = np.array([pts[list(c), 1].mean() for c in cells])
+    nv = np.where(cells[:, 3] >= 0, 4.0, 3.0)
+    idx = np.where(cells >= 0, cells, 0)
+    w = (cells >= 0).astype(np.float64)
+    cx = (pts[idx, 0] * w).sum(1) / nv
+    cy = (pts[idx, 1] * w).sum(1) / nv
     mx = 0.5 * (pts[n1, 0] + pts[n2, 0])
     my = 0.5 * (pts[n1, 1] + pts[n2, 1])
     bnd = valid & (NAC == 0)

@@ -123,7 +123,9 @@ typedef struct hydro_tune {
     int32_t graph_chunk;    /* steps per captured hipGraph (GRAPH modes) */
     int32_t persistent_wgs_per_cu; /* 0 = occupancy maximum */
     int32_t materialize_flux; /* 1: FUSED modes also write FLUX0..3 every step */
-    int32_t reserved[2];
+    int32_t waves_per_simd; /* 0: compiler's choice; 4: fused kernels limited to 128 VGPRs so
+                               4 waves fit per SIMD (one resident round at 207K cells) */
+    int32_t reserved;
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -184,6 +186,22 @@ int hydro_dump(hydro_mesh *m, const char *path, hipStream_t stream);
  * Enabled by hydro_set_profiling(m, 1); adds one event pair per launch. */
 int hydro_set_profiling(hydro_mesh *m, int enable);
 int hydro_last_timing(const hydro_mesh *m, double *total_ms, int *launches, int *steps);
+
+/* ---- batches of meshes ---------------------------------------------------- */
+/* No counterpart in the reference (it runs one mesh per process): many
+ * meshes of uneven size, or many scenarios as separate meshes, advanced in
+ * lockstep by one launch per step (FUSED), a replayed hipGraph (GRAPH) or a
+ * single persistent launch (PERSISTENT).  Workgroup tiles are packed CSR-style
+ * (each tile belongs to one mesh), so small meshes share the chip instead of
+ * each launching a mostly idle grid.  Every mesh keeps its own buffers and
+ * time cursor; all meshes need the same dtype and variant, and are
+ * (re)prepared with `tune`.  A mesh must not be stepped on its own while it
+ * belongs to a live batch on another stream. */
+typedef struct hydro_batch hydro_batch;
+int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune, hydro_batch **out);
+int hydro_batch_destroy(hydro_batch *b);
+int hydro_batch_step_n(hydro_batch *b, int nsteps, int mode, hipStream_t stream);
+int hydro_batch_sync(hydro_batch *b, hipStream_t stream);
 
 /* ---- adjoint and tangent-linear steps (fp64, variant A) ------------------- */
 /* No counterpart in the reference: these make gradients of a gauge misfit
