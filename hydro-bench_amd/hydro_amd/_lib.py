@@ -112,6 +112,11 @@ SIGNATURES = {
     "hydro_cells_map": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "hydro_states_map": (c_int, [c_void_p] * 6 + [c_int, c_void_p]),
     "hydro_refresh_statics": (c_int, [c_void_p, c_void_p]),
+    # batches
+    "hydro_batch_create": (c_int, [c_void_p, c_int, POINTER(Tune), POINTER(c_void_p)]),
+    "hydro_batch_destroy": (c_int, [c_void_p]),
+    "hydro_batch_step_n": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "hydro_batch_sync": (c_int, [c_void_p, c_void_p]),
 }
 
 _lib = None
