@@ -416,7 +416,7 @@ __host__ __device__ inline UpdateBar update_bar(const Scalars<double> &S, double
         if (fabs(V1) < K::VMIN) b.V1 += lV;
     } else {
         const double sp = sqrt(U1 * U1 + V1 * V1);
-        const double ph = pow(H1, 0.33333);
+        const double ph = pow_d(H1, 0.33333);
         const double WSF = FNC * sp / ph;
         const double NU = H1 * U1 - WDTA * WU - S.DT * WSF * U1;
         const double NV = H1 * V1 - WDTA * WV - S.DT * WSF * V1;

@@ -97,14 +97,18 @@ struct hydro_mesh {
     hydro_tune tune{};
     int block_two = 256;
 
-    // kinds present (from KLAS at create time)
+    // kinds present (from KLAS at create time; KLAS codes are fixed for the
+    // mesh's life except KLAS 7 -> 0, which only removes a kind)
     bool has_kind[16] = {};
+    bool rare = false;   // a KLAS 3/5/6/7 or unknown code is present
 
     // reordered layout
     bool prepared = false;
     int cur = 0;  // state record buffer holding the current state
     bool ext_valid = true, int_valid = false;
     DevBuf st[2], W, ZBC, AREA, FNC, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
+    DevBuf task_off, task_a, task_b;   // TILES scheme task lists
+    int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
     DevBuf tick, barrier, perr, ext_dev;
     bool need_zb1 = false, need_slcs = false;
@@ -172,8 +176,12 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.SLCOS = m->need_slcs ? m->SLCOS.as<R>() : nullptr;
     f.SLSIN = m->need_slcs ? m->SLSIN.as<R>() : nullptr;
     f.orig = m->orig.as<int32_t>();
+    f.task_off = m->task_off.as<int32_t>();
+    f.task_a = m->task_a.as<int32_t>();
+    f.task_b = m->task_b.as<int32_t>();
     f.materialize = m->tune.materialize_flux;
-    f.occ4 = m->tune.waves_per_simd == 4;
+    f.scheme = m->tune.scheme;
+    f.rare = m->rare;
     f.rec_state = nullptr;
     f.rec_wsum = nullptr;
     return f;
@@ -209,6 +217,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
     std::vector<int32_t> perm;
     if (m->tune.reorder == HYDRO_REORDER_RCM) {
         perm = rcm_order(C, NAC.data());
+    } else if (m->tune.reorder == HYDRO_REORDER_TILES) {
+        perm = tile_order(C, NAC.data(), m->tune.block, rcm_order(C, NAC.data()));
     } else if (m->tune.reorder == HYDRO_REORDER_NONE) {
         perm.resize(C);
         for (int i = 0; i < C; i++) perm[i] = i;
@@ -273,6 +283,65 @@ template <class R> int prepare_t(hydro_mesh *m) {
                 sls_n[sn] = SLSIN[so];
             }
         }
+    }
+    // TILES scheme tasks.  An interior edge whose two slots lie in one tile,
+    // one primary and one mirror, with exactly opposite normals, is one pair
+    // task (listed from its primary slot); every other non-padding slot is a
+    // one-slot task.  Within a tile: pairs, then interior slots, then
+    // boundary slots, so most waves run one code path.
+    {
+        const int B = m->tune.block;
+        const int ntiles = (C + B - 1) / B;
+        std::vector<int32_t> toff(ntiles + 1, 0), ta, tb, sgl, bnd;
+        ta.reserve(S4);
+        tb.reserve(S4);
+        int npairs = 0, nsingles = 0;
+        auto neg_equal = [](R a, R b) {
+            const R na = -a;
+            return std::memcmp(&na, &b, sizeof(R)) == 0;
+        };
+        for (int t = 0; t < ntiles; t++) {
+            toff[t] = (int32_t)ta.size();
+            sgl.clear();
+            bnd.clear();
+            const int pend = std::min(C, (t + 1) * B);
+            for (int p = t * B; p < pend; p++)
+                for (int j = 0; j < 4; j++) {
+                    const size_t sn = (size_t)j * C + p;
+                    const int kind = meta_n[sn] & META_KIND_MASK;
+                    if (kind == K_PAD) continue;
+                    if (kind >= K_B10) {
+                        bnd.push_back(4 * p + j);
+                        continue;
+                    }
+                    const int n = nbr_n[sn];
+                    if (n >= t * B && n < pend && twin_n[sn] != 0xff) {
+                        const int k = twin_n[sn];
+                        const size_t sk = (size_t)k * C + n;
+                        const int kn = meta_n[sk] & META_KIND_MASK;
+                        const bool pairable = twin_n[sk] == j && nbr_n[sk] == p &&
+                                              ((kind == K_PRIMARY && kn == K_MIRROR) ||
+                                               (kind == K_MIRROR && kn == K_PRIMARY)) &&
+                                              neg_equal(cos_n[sn], cos_n[sk]) && neg_equal(sin_n[sn], sin_n[sk]);
+                        if (pairable) {
+                            if (kind == K_PRIMARY) {
+                                ta.push_back(4 * p + j);
+                                tb.push_back(4 * n + k);
+                                npairs++;
+                            }
+                            continue;
+                        }
+                    }
+                    sgl.push_back(4 * p + j);
+                }
+            for (int32_t w : sgl) { ta.push_back(w); tb.push_back(-1); }
+            for (int32_t w : bnd) { ta.push_back(w); tb.push_back(-1); }
+            nsingles += (int)(sgl.size() + bnd.size());
+        }
+        toff[ntiles] = (int32_t)ta.size();
+        m->n_pairs = npairs;
+        m->n_singles = nsingles;
+        if ((rc = h2d(m->task_off, toff)) || (rc = h2d(m->task_a, ta)) || (rc = h2d(m->task_b, tb))) return rc;
     }
     if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
         (rc = h2d(m->ZB1, zb1_n)) || (rc = h2d(m->nbr, nbr_n)) || (rc = h2d(m->meta, meta_n)) ||
@@ -420,9 +489,9 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
     int rc = ensure_internal<R>(m, s);
     if (rc) return rc;
     if (n == 0) return HYDRO_OK;
-    const int block = 256;
+    const int block = m->tune.block;
     int maxg = 0;
-    HIP_TRY((persistent_max_grid<R, VAR>(block, &maxg)));
+    HIP_TRY((persistent_max_grid<R, VAR>(block, m->rare, m->tune.scheme, &maxg)));
     const int ntiles = (m->CELL + block - 1) / block;
     int want = m->tune.persistent_wgs_per_cu;
     int grid = maxg;
@@ -465,12 +534,13 @@ template <class R, int VAR> int step_dispatch(hydro_mesh *m, int n, int mode, hi
 hydro_tune default_tune() {
     hydro_tune t{};
     t.block = 256;
-    t.reorder = HYDRO_REORDER_RCM;
+    t.reorder = HYDRO_REORDER_TILES;
     t.xcd_remap = 1;
     t.graph_chunk = 16;
     t.persistent_wgs_per_cu = 0;
     t.materialize_flux = 0;
     t.waves_per_simd = 0;
+    t.scheme = HYDRO_SCHEME_TILES;
     return t;
 }
 
@@ -528,6 +598,8 @@ int hydro_mesh_create(const hydro_mesh_desc *desc, hydro_mesh **out) {
             return fail(HYDRO_ERR_INVALID, "NAC[%zu] = %d outside [0, CELL=%d]", s, NAC[s], d.CELL);
         m->has_kind[kind_of_klas(KLAS[s], false)] = true;
     }
+    m->rare = m->has_kind[K_B3] || m->has_kind[K_B5] || m->has_kind[K_B6] || m->has_kind[K_B7] ||
+              m->has_kind[K_BUNK];
     if (m->has_kind[K_B10] && (!d.QT || !d.DQT)) return fail(HYDRO_ERR_INVALID, "KLAS 10 slots need QT and DQT");
     if (m->has_kind[K_B1] && (!d.ZT || !d.DZT)) return fail(HYDRO_ERR_INVALID, "KLAS 1 slots need ZT and DZT");
     if (m->has_kind[K_B3] && (!d.ZW || !d.QW || !d.NHQ1 || d.NHQ < 2))
@@ -555,9 +627,11 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     hydro_tune t = tune ? *tune : default_tune();
     if (t.block < 64 || t.block > kFusedMaxBlock || (t.block % 64))
         return fail(HYDRO_ERR_INVALID, "block %d (64..%d, multiple of 64)", t.block, kFusedMaxBlock);
-    if (t.reorder < 0 || t.reorder > 2) return fail(HYDRO_ERR_INVALID, "reorder %d", t.reorder);
-    if (t.waves_per_simd != 0 && t.waves_per_simd != 4)
-        return fail(HYDRO_ERR_INVALID, "waves_per_simd %d (0 or 4)", t.waves_per_simd);
+    if (t.reorder < 0 || t.reorder > 3) return fail(HYDRO_ERR_INVALID, "reorder %d", t.reorder);
+    if (t.waves_per_simd != 0)
+        return fail(HYDRO_ERR_INVALID, "waves_per_simd %d: only 0 (the compiler's choice)", t.waves_per_simd);
+    if (t.scheme != HYDRO_SCHEME_TILES && t.scheme != HYDRO_SCHEME_CELL)
+        return fail(HYDRO_ERR_INVALID, "scheme %d", t.scheme);
     // the internal state must be written back before the layout is replaced
     int rc = HYDRO_OK;
     if (m->prepared && !m->ext_valid)
@@ -889,6 +963,8 @@ struct hydro_batch {
     std::vector<hydro_mesh *> meshes;
     int dtype = 0, variant = 0, block = 256, remap = 1, chunk = 16;
     int ntiles = 0;
+    bool rare = false;
+    int scheme = 0;
     DevBuf views, tile_mesh, tile0, g0, spd, tick, barrier, perr;
     std::vector<int32_t> h_tile0;
     hipStream_t cap_stream = nullptr;
@@ -930,6 +1006,8 @@ template <class R> BatchView<R> batch_view(const hydro_batch *b) {
     v.g0 = b->g0.as<int64_t>();
     v.spd = b->spd.as<int32_t>();
     v.ntiles = b->ntiles;
+    v.rare = b->rare;
+    v.scheme = b->scheme;
     return v;
 }
 
@@ -980,7 +1058,7 @@ template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hip
     } else if (mode == HYDRO_MODE_PERSISTENT) {
         if (n > 0) {
             int maxg = 0;
-            HIP_TRY((batch_persistent_max_grid<R, VAR>(b->block, &maxg)));
+            HIP_TRY((batch_persistent_max_grid<R, VAR>(b->block, b->rare, b->scheme, &maxg)));
             const int grid = std::max(1, std::min(maxg, b->ntiles));
             PersistentArgs pa;
             pa.nsteps = n;
@@ -1013,6 +1091,7 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
     b->block = t.block;
     b->remap = t.xcd_remap;
     b->chunk = t.graph_chunk;
+    b->scheme = t.scheme;
     std::vector<int32_t> tile_mesh, tile0(n);
     for (int i = 0; i < n; i++) {
         hydro_mesh *m = meshes[i];
@@ -1022,7 +1101,7 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
         for (int j = 0; j < i; j++)
             if (meshes[j] == m) return fail(HYDRO_ERR_INVALID, "mesh %d appears twice", i);
         if (!m->prepared || m->tune.block != t.block || m->tune.reorder != t.reorder ||
-            m->tune.materialize_flux != t.materialize_flux) {
+            m->tune.materialize_flux != t.materialize_flux || m->tune.scheme != t.scheme) {
             int rc = hydro_prepare(m, &t);
             if (rc) return rc;
         }
@@ -1030,6 +1109,7 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
         const int nt = (m->CELL + t.block - 1) / t.block;
         for (int k = 0; k < nt; k++) tile_mesh.push_back(i);
         b->meshes.push_back(m);
+        b->rare = b->rare || m->rare;
     }
     b->ntiles = (int)tile_mesh.size();
     b->h_tile0 = tile0;
@@ -1099,6 +1179,15 @@ int hydro_rcm_order(int CELL, const int32_t *NAC, int32_t *perm) {
     for (int i = 0; i < 4 * CELL; i++)
         if (NAC[i] < 0 || NAC[i] > CELL) return fail(HYDRO_ERR_INVALID, "NAC[%d] = %d", i, NAC[i]);
     std::vector<int32_t> p = rcm_order(CELL, NAC);
+    std::memcpy(perm, p.data(), sizeof(int32_t) * CELL);
+    return HYDRO_OK;
+}
+
+int hydro_tile_order(int CELL, const int32_t *NAC, int block, int32_t *perm) {
+    if (CELL < 0 || (!NAC && CELL) || (!perm && CELL) || block < 1) return fail(HYDRO_ERR_INVALID, "bad arguments");
+    for (int i = 0; i < 4 * CELL; i++)
+        if (NAC[i] < 0 || NAC[i] > CELL) return fail(HYDRO_ERR_INVALID, "NAC[%d] = %d", i, NAC[i]);
+    std::vector<int32_t> p = tile_order(CELL, NAC, block, rcm_order(CELL, NAC));
     std::memcpy(perm, p.data(), sizeof(int32_t) * CELL);
     return HYDRO_OK;
 }

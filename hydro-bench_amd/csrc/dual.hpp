@@ -9,6 +9,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "pow_fixed.hpp"
+
 namespace hydro {
 namespace ad {
 
@@ -82,7 +84,7 @@ __host__ __device__ inline Dual copysign(Dual a, double s) { return copysign(a, 
 __host__ __device__ inline Dual copysign(double a, Dual s) { return copysign(Dual(a), s); }
 // x^y for constant y
 __host__ __device__ inline Dual pow_d(Dual x, double y) {
-    const double p = ::pow(x.v, y);
+    const double p = hydro::pow_d(x.v, y);
     return Dual(p, x.v != 0 ? y * p / x.v * x.d : 0.0);
 }
 __host__ __device__ inline Dual pow_rare(Dual x, double y) { return pow_d(x, y); }

@@ -10,6 +10,7 @@
 //  k_persistent : the fused step for many steps in one launch, with a
 //      grid-wide barrier between steps (the design of
 //      cuda_impl/F2_driver.cu:34-44, written natively).
+#include "ext_step.hpp"
 #include "kernels.hpp"
 
 namespace hydro {
@@ -26,83 +27,27 @@ __device__ __forceinline__ int tile_of(int wg, int nwg, int remap) {
     return x * q + (x < r ? x : r) + k;
 }
 
-template <class R>
-__device__ __forceinline__ BcCtx<R> make_bc(const ExtView<R> &m, int cell, int slot, int kt, int jt) {
-    BcCtx<R> bc;
-    bc.QT = m.QT; bc.DQT = m.DQT; bc.ZT = m.ZT; bc.DZT = m.DZT;
-    bc.ZW = m.ZW; bc.QW = m.QW; bc.BF = m.BF; bc.NHQ1 = m.NHQ1;
-    bc.CELL = m.CELL; bc.NHQ = m.NHQ; bc.cell = cell; bc.kt = kt; bc.jt = jt;
-    bc.SIDE = m.SIDE[slot]; bc.COSF = m.COSF[slot]; bc.SINF = m.SINF[slot];
-    return bc;
-}
-
-template <class R> __device__ __forceinline__ BcCtx<R> empty_bc() {
-    BcCtx<R> bc;
-    bc.QT = bc.DQT = bc.ZT = bc.DZT = bc.ZW = bc.QW = bc.BF = nullptr;
-    bc.NHQ1 = nullptr;
-    bc.CELL = bc.NHQ = bc.cell = bc.kt = bc.jt = 0;
-    bc.SIDE = bc.COSF = bc.SINF = 0;
-    return bc;
-}
-
 // ---------------------------------------------------------------------------
 // Two-kernel path on the caller's layout.
 
 template <class R, int VAR>
 __global__ void __launch_bounds__(kMaxBlock) k_calculate_flux(ExtView<R> m, int kt, int jt) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= 4 * m.CELL) return;
-    const int pos = idx >> 2;
-    const R H1 = m.H[pos], U1 = m.U[pos], V1 = m.V[pos], Z1 = m.Z[pos];
-    const R ZB1 = m.ZB1[pos], ZBp = m.ZBC[pos];
-    const int NC = m.NAC[idx] - 1;
-    const int klas = m.KLAS[idx];
-    const int kind = kind_of_klas(klas, pos < NC);
-    const bool has = (NC != -1);
-    R Hn = 0, Un = 0, Vn = 0, Zn = 0, ZBn = 0;
-    if (has) { Hn = m.H[NC]; Un = m.U[NC]; Vn = m.V[NC]; Zn = m.Z[NC]; ZBn = m.ZBC[NC]; }
-    Flux<R> prev = {0, 0, 0, 0};
-    BcCtx<R> bc = empty_bc<R>();
-    if (kind >= K_B10) {
-        bc = make_bc<R>(m, pos, idx, kt, jt);
-        if (kind == K_BUNK) prev = {m.F0[idx], m.F1[idx], m.F2[idx], m.F3[idx]};
-    }
-    bool fired = false;
-    const Flux<R> f = slot_flux<R, VAR>(kind, m.S, bc, H1, U1, V1, Z1, ZB1, ZBp, has, Hn, Un, Vn,
-                                        Zn, ZBn, m.COSF[idx], m.SINF[idx], prev, &fired);
-    if (fired) m.KLAS[idx] = 0;
-    m.F0[idx] = f.f0;
-    m.F1[idx] = f.f1;
-    m.F2[idx] = f.f2;
-    m.F3[idx] = f.f3;
+    if (idx < 4 * m.CELL) ext_flux_slot<R, VAR>(m, idx, kt, jt);
 }
 
 template <class R, int VAR>
 __global__ void __launch_bounds__(kMaxBlock) k_update_cell(ExtView<R> m) {
     const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos >= m.CELL) return;
-    R WH = 0.0, WU = 0.0, WV = 0.0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int idx = 4 * pos + j;
-        const Flux<R> f = {m.F0[idx], m.F1[idx], m.F2[idx], m.F3[idx]};
-        accumulate<R>(WH, WU, WV, f, m.SIDE[idx], m.SLCOS[idx], m.SLSIN[idx]);
-    }
-    const int b = 4 * pos;
-    const CellOut<R> o = update_math<R, VAR>(m.S, m.H[pos], m.U[pos], m.V[pos], m.ZBC[pos], m.AREA[pos],
-                                             m.FNC[pos], WH, WU, WV, m.SIDE[b], m.SIDE[b + 1],
-                                             m.SIDE[b + 2], m.SIDE[b + 3]);
-    m.H[pos] = o.H;
-    m.U[pos] = o.U;
-    m.V[pos] = o.V;
-    m.Z[pos] = o.Z;
-    m.W[pos] = o.W;
+    if (pos < m.CELL) ext_update_cell<R, VAR>(m, pos);
 }
 
 // ---------------------------------------------------------------------------
 // Fused step on the reordered layout.
 
-template <class R, int VAR>
+// RARE = false: the mesh has only interior, KLAS 10/1/4 and padding slots
+// (checked on the host), so the other boundary handlers are not compiled in.
+template <class R, int VAR, bool RARE>
 __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R> *__restrict__ Sin,
                                            State<R> *__restrict__ Sout, int p, int kt, int jt) {
     const int C = f.C;
@@ -123,7 +68,7 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
         if (kind == K_PAD) {
             if (f.materialize) {
                 bool fired = false;
-                fl = slot_flux<R, VAR>(K_MIRROR, S, empty_bc<R>(), me.H, me.U, me.V, me.Z, ZB1, ZBp,
+                fl = slot_flux<R, VAR, RARE>(K_MIRROR, S, empty_bc<R>(), me.H, me.U, me.V, me.Z, ZB1, ZBp,
                                        false, 0, 0, 0, 0, 0, (R)0, (R)0, Flux<R>{0, 0, 0, 0}, &fired);
             } else {
                 fl = padding_flux_signs<R>(S, me.H, me.U, me.V, me.Z, ZB1, ZBp);
@@ -145,12 +90,12 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
                 const ExtView<R> &m = *f.ext;
                 const int eo = 4 * f.orig[p] + j;
                 bc = make_bc<R>(m, f.orig[p], eo, kt, jt);
-                if (kind == K_BUNK) prev = {m.F0[eo], m.F1[eo], m.F2[eo], m.F3[eo]};
+                if (RARE && kind == K_BUNK) prev = {m.F0[eo], m.F1[eo], m.F2[eo], m.F3[eo]};
             }
             bool fired = false;
-            fl = slot_flux<R, VAR>(kind, S, bc, me.H, me.U, me.V, me.Z, ZB1, ZBp, has, o.H, o.U, o.V,
+            fl = slot_flux<R, VAR, RARE>(kind, S, bc, me.H, me.U, me.V, me.Z, ZB1, ZBp, has, o.H, o.U, o.V,
                                    o.Z, ZBn, c, s, prev, &fired);
-            if (fired) {
+            if (RARE && fired) {
                 f.meta[e] = (mt & META_PRIMARY_BIT) ? (uint8_t)(K_PRIMARY | META_PRIMARY_BIT)
                                                     : (uint8_t)K_MIRROR;
                 f.ext->KLAS[4 * f.orig[p] + j] = 0;
@@ -159,7 +104,10 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
             const R SLSA = f.SLSIN ? f.SLSIN[e] : SIDE * s;
             accumulate<R>(WH, WU, WV, fl, SIDE, SLCA, SLSA);
         }
-        if (f.materialize) {
+        // A slot whose code has no handler keeps FLUX0/1/3 from its previous
+        // step (and the supercritical branch writes all four), so its flux
+        // always goes to the caller's arrays.
+        if (f.materialize || (RARE && kind == K_BUNK)) {
             const ExtView<R> &m = *f.ext;
             const int eo = 4 * f.orig[p] + j;
             m.F0[eo] = fl.f0; m.F1[eo] = fl.f1; m.F2[eo] = fl.f2; m.F3[eo] = fl.f3;
@@ -177,22 +125,180 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
     f.W[p] = o.W;
 }
 
-template <class R, int VAR>
-__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt, int jt, int remap) {
-    const int t = tile_of(blockIdx.x, gridDim.x, remap);
-    const int p = t * blockDim.x + threadIdx.x;
-    if (p >= f.C) return;
-    fused_cell<R, VAR>(f, f.Sin, f.Sout, p, kt, jt);
+// ---------------------------------------------------------------------------
+// TILES scheme: one workgroup per tile of blockDim.x consecutive cells.
+//  1. stage the tile's cell states and beds in LDS;
+//  2. run the tile's tasks: an edge joining two cells of the tile is solved
+//     once for both slots (pair_flux), any other slot alone (slot_flux);
+//     each slot's three flux-sum terms go to LDS;
+//  3. each thread sums its cell's four slots in slot order (as the
+//     reference's UpdateCellKernel does, F:732-742) and updates the cell.
+// Phase 3 reads only LDS terms and registers, so a following tile's phase 1
+// cannot overwrite anything still in use once phase 2's barrier is passed.
+
+template <class R> struct TileLds {
+    State<R> *st;   // [B]
+    R *zb;          // [B]
+    R *c;           // [3][4][B]: WH, WU, WV terms of slot j of local cell l
+};
+
+template <class R> __device__ __forceinline__ TileLds<R> tile_lds(int B) {
+    extern __shared__ __align__(32) unsigned char hydro_tile_smem[];
+    TileLds<R> L;
+    L.st = reinterpret_cast<State<R> *>(hydro_tile_smem);
+    L.zb = reinterpret_cast<R *>(hydro_tile_smem + sizeof(State<R>) * B);
+    L.c = L.zb + B;
+    return L;
 }
 
-// Same step limited to 128 VGPRs so four waves fit on each SIMD.
-template <class R, int VAR>
-__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_fused_step_o4(FusedView<R> f, int kt, int jt, int remap) {
-    const int t = tile_of(blockIdx.x, gridDim.x, remap);
-    const int p = t * blockDim.x + threadIdx.x;
-    if (p >= f.C) return;
-    fused_cell<R, VAR>(f, f.Sin, f.Sout, p, kt, jt);
+// Writes the slot's terms to LDS; the flux itself goes to the caller's
+// FLUX0..3 when materialising, or when `keep` (a slot whose code has no
+// handler reads its previous FLUX0/1/3 next step).
+template <class R>
+__device__ __forceinline__ void tile_store(const FusedView<R> &f, const TileLds<R> &L, int B, int l, int j, int e,
+                                           int p, const Flux<R> &fl, R c, R s, bool keep = false) {
+    const R SIDE = f.SIDE[e];
+    const R SLCA = f.SLCOS ? f.SLCOS[e] : SIDE * c;
+    const R SLSA = f.SLSIN ? f.SLSIN[e] : SIDE * s;
+    const Contrib<R> t = contrib<R>(fl, SIDE, SLCA, SLSA);
+    L.c[j * B + l] = t.h;
+    L.c[(4 + j) * B + l] = t.u;
+    L.c[(8 + j) * B + l] = t.v;
+    if (f.materialize || keep) {
+        const ExtView<R> &m = *f.ext;
+        const int eo = 4 * f.orig[p] + j;
+        m.F0[eo] = fl.f0; m.F1[eo] = fl.f1; m.F2[eo] = fl.f2; m.F3[eo] = fl.f3;
+    }
+}
+
+template <class R, int VAR, bool RARE>
+__device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
+                                           State<R> *__restrict__ Sout, int t, int kt, int jt) {
+    const int B = blockDim.x, C = f.C, i = threadIdx.x;
+    const int p0 = t * B, p = p0 + i;
+    const bool live = p < C;
+    const Scalars<R> S = f.S;
+    const TileLds<R> L = tile_lds<R>(B);
+    State<R> me = {0, 0, 0, 0};
+    R zbp = 0;
+    if (live) {
+        me = Sin[p];
+        zbp = f.ZBC[p];
+        L.st[i] = me;
+        L.zb[i] = zbp;
+    }
+    __syncthreads();
+    const int k1 = f.task_off[t + 1];
+    for (int k = f.task_off[t] + i; k < k1; k += B) {
+        const int wa = f.task_a[k], wb = f.task_b[k];
+        const int pa = wa >> 2, ja = wa & 3, la = pa - p0;
+        const int ea = ja * C + pa;
+        const State<R> A = L.st[la];
+        const R zbA = L.zb[la];
+        const R zb1A = f.ZB1 ? f.ZB1[pa] : (R)(zbA + S.HM1);
+        const R ca = f.COSF[ea], sa = f.SINF[ea];
+        if (wb >= 0) {
+            const int pb = wb >> 2, jb = wb & 3, lb = pb - p0;
+            const int eb = jb * C + pb;
+            const State<R> M = L.st[lb];
+            const R zbM = L.zb[lb];
+            const R zb1M = f.ZB1 ? f.ZB1[pb] : (R)(zbM + S.HM1);
+            Flux<R> fA, fM;
+            pair_flux<R>(S, A.H, A.U, A.V, A.Z, zb1A, zbA, M.H, M.U, M.V, M.Z, zb1M, zbM, ca, sa, &fA, &fM);
+            tile_store<R>(f, L, B, la, ja, ea, pa, fA, ca, sa);
+            tile_store<R>(f, L, B, lb, jb, eb, pb, fM, f.COSF[eb], f.SINF[eb]);
+        } else {
+            const int mt = f.meta[ea];
+            const int kind = mt & META_KIND_MASK;
+            const int nb = f.nbr[ea];
+            const bool has = nb >= 0;
+            State<R> o = {0, 0, 0, 0};
+            R ZBn = 0;
+            if (has) {
+                const unsigned ln = (unsigned)(nb - p0);
+                if (ln < (unsigned)B) {
+                    o = L.st[ln];
+                    ZBn = L.zb[ln];
+                } else {
+                    o = Sin[nb];
+                    ZBn = f.ZBC[nb];
+                }
+            }
+            Flux<R> prev = {0, 0, 0, 0};
+            BcCtx<R> bc = empty_bc<R>();
+            if (kind >= K_B10) {
+                const ExtView<R> &m = *f.ext;
+                const int eo = 4 * f.orig[pa] + ja;
+                bc = make_bc<R>(m, f.orig[pa], eo, kt, jt);
+                if (RARE && kind == K_BUNK) prev = {m.F0[eo], m.F1[eo], m.F2[eo], m.F3[eo]};
+            }
+            bool fired = false;
+            const Flux<R> fl = slot_flux<R, VAR, RARE>(kind, S, bc, A.H, A.U, A.V, A.Z, zb1A, zbA, has, o.H, o.U,
+                                                       o.V, o.Z, ZBn, ca, sa, prev, &fired);
+            if (RARE && fired) {
+                f.meta[ea] = (mt & META_PRIMARY_BIT) ? (uint8_t)(K_PRIMARY | META_PRIMARY_BIT)
+                                                     : (uint8_t)K_MIRROR;
+                f.ext->KLAS[4 * f.orig[pa] + ja] = 0;
+            }
+            tile_store<R>(f, L, B, la, ja, ea, pa, fl, ca, sa, RARE && kind == K_BUNK);
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    R WH = 0.0, WU = 0.0, WV = 0.0;
+    R sd[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int e = j * C + p;
+        const int kind = f.meta[e] & META_KIND_MASK;
+        sd[j] = f.SIDE[e];
+        if (kind == K_PAD) {
+            const R ZB1 = f.ZB1 ? f.ZB1[p] : (R)(zbp + S.HM1);
+            Flux<R> fl;
+            if (f.materialize) {
+                bool fired = false;
+                fl = slot_flux<R, VAR, RARE>(K_MIRROR, S, empty_bc<R>(), me.H, me.U, me.V, me.Z, ZB1, zbp, false,
+                                             0, 0, 0, 0, 0, (R)0, (R)0, Flux<R>{0, 0, 0, 0}, &fired);
+                const ExtView<R> &m = *f.ext;
+                const int eo = 4 * f.orig[p] + j;
+                m.F0[eo] = fl.f0; m.F1[eo] = fl.f1; m.F2[eo] = fl.f2; m.F3[eo] = fl.f3;
+            } else {
+                fl = padding_flux_signs<R>(S, me.H, me.U, me.V, me.Z, ZB1, zbp);
+            }
+            accumulate<R>(WH, WU, WV, fl, (R)0, (R)0, (R)0);
+        } else {
+            WH += L.c[j * B + i];
+            WU += L.c[(4 + j) * B + i];
+            WV += L.c[(8 + j) * B + i];
+        }
+    }
+    if (f.rec_wsum) {
+        f.rec_state[p] = me;
+        f.rec_wsum[p] = WH;
+        f.rec_wsum[C + p] = WU;
+        f.rec_wsum[2 * C + p] = WV;
+    }
+    const CellOut<R> o = update_math<R, VAR>(S, me.H, me.U, me.V, zbp, f.AREA[p], f.FNC[p], WH, WU, WV, sd[0],
+                                             sd[1], sd[2], sd[3]);
+    Sout[p] = State<R>{o.H, o.U, o.V, o.Z};
+    f.W[p] = o.W;
+}
+
+// One step of tile t under either scheme.
+template <class R, int VAR, bool RARE, int SCHEME>
+__device__ __forceinline__ void step_tile(const FusedView<R> &f, const State<R> *__restrict__ Sin,
+                                          State<R> *__restrict__ Sout, int t, int kt, int jt) {
+    if (SCHEME == 0) {
+        tile_cells<R, VAR, RARE>(f, Sin, Sout, t, kt, jt);
+    } else {
+        const int p = t * blockDim.x + threadIdx.x;
+        if (p < f.C) fused_cell<R, VAR, RARE>(f, Sin, Sout, p, kt, jt);
+    }
+}
+
+template <class R, int VAR, bool RARE, int SCHEME>
+__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt, int jt, int remap) {
+    step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, jt);
 }
 
 // (day, kt) of global step g counted from day 0, kt 1 (benchmark.cu:89-90).
@@ -205,26 +311,12 @@ __device__ __forceinline__ void time_of(int64_t g, int spd, int *day, int *kt) {
 // Graph-replayable variant: the step's time comes from a device tick, so the
 // same captured graph advances the boundary series on every replay (the
 // reference's graph replays one frozen (kt=1, day=0) step, benchmark.cu:235-240).
-template <class R, int VAR>
+template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_tick(FusedView<R> f, const int64_t *tick,
                                                                     int offset, int spd, int remap) {
-    const int t = tile_of(blockIdx.x, gridDim.x, remap);
-    const int p = t * blockDim.x + threadIdx.x;
-    if (p >= f.C) return;
     int day, kt;
     time_of(*tick + offset, spd, &day, &kt);
-    fused_cell<R, VAR>(f, f.Sin, f.Sout, p, kt, day);
-}
-
-template <class R, int VAR>
-__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_fused_step_tick_o4(FusedView<R> f, const int64_t *tick, int offset, int spd, int remap) {
-    const int t = tile_of(blockIdx.x, gridDim.x, remap);
-    const int p = t * blockDim.x + threadIdx.x;
-    if (p >= f.C) return;
-    int day, kt;
-    time_of(*tick + offset, spd, &day, &kt);
-    fused_cell<R, VAR>(f, f.Sin, f.Sout, p, kt, day);
+    step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, day);
 }
 
 __global__ void k_tick_advance(int64_t *tick, int by) {
@@ -260,7 +352,7 @@ __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target,
     __syncthreads();
 }
 
-template <class R, int VAR>
+template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_persistent(FusedView<R> f, PersistentArgs pa) {
     const int ntiles = (f.C + blockDim.x - 1) / blockDim.x;
     const int64_t g0 = (int64_t)pa.day0 * (pa.spd - 1) + (pa.kt0 - 1);
@@ -269,10 +361,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_persistent(FusedView<R> f, P
     for (int s = 0; s < pa.nsteps; s++) {
         int day, kt;
         time_of(g0 + s, pa.spd, &day, &kt);
-        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            const int p = t * blockDim.x + threadIdx.x;
-            if (p < f.C) fused_cell<R, VAR>(f, a, b, p, kt, day);
-        }
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) step_tile<R, VAR, RARE, SCHEME>(f, a, b, t, kt, day);
         State<R> *tmp = const_cast<State<R> *>(a);
         a = b;
         b = tmp;
@@ -286,33 +375,32 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_persistent(FusedView<R> f, P
 // ---------------------------------------------------------------------------
 // Batches of meshes.
 
-template <class R, int VAR>
+template <class R, int VAR, bool RARE, int SCHEME>
 __device__ __forceinline__ void batch_tile(const BatchView<R> &b, int t, int step) {
     const int m = __builtin_amdgcn_readfirstlane(b.tile_mesh[t]);
     const FusedView<R> &f = b.views[m];
-    const int p = (t - b.tile0[m]) * blockDim.x + threadIdx.x;
-    if (p >= f.C) return;
     int day, kt;
     time_of(b.g0[m] + step, b.spd[m], &day, &kt);
     const bool odd = step & 1;
-    fused_cell<R, VAR>(f, odd ? f.Sout : f.Sin, odd ? const_cast<State<R> *>(f.Sin) : f.Sout, p, kt, day);
+    step_tile<R, VAR, RARE, SCHEME>(f, odd ? f.Sout : f.Sin, odd ? const_cast<State<R> *>(f.Sin) : f.Sout,
+                                    t - b.tile0[m], kt, day);
 }
 
-template <class R, int VAR>
+template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_batch_step(BatchView<R> b, int step, int remap) {
-    batch_tile<R, VAR>(b, tile_of(blockIdx.x, gridDim.x, remap), step);
+    batch_tile<R, VAR, RARE, SCHEME>(b, tile_of(blockIdx.x, gridDim.x, remap), step);
 }
 
-template <class R, int VAR>
+template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_batch_step_tick(BatchView<R> b, const int64_t *tick, int offset,
                                                                     int remap) {
-    batch_tile<R, VAR>(b, tile_of(blockIdx.x, gridDim.x, remap), (int)(*tick) + offset);
+    batch_tile<R, VAR, RARE, SCHEME>(b, tile_of(blockIdx.x, gridDim.x, remap), (int)(*tick) + offset);
 }
 
-template <class R, int VAR>
+template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_batch_persistent(BatchView<R> b, PersistentArgs pa) {
     for (int s = 0; s < pa.nsteps; s++) {
-        for (int t = blockIdx.x; t < b.ntiles; t += gridDim.x) batch_tile<R, VAR>(b, t, s);
+        for (int t = blockIdx.x; t < b.ntiles; t += gridDim.x) batch_tile<R, VAR, RARE, SCHEME>(b, t, s);
         if (s + 1 < pa.nsteps) {
             grid_barrier(pa.barrier, (unsigned)(s + 1) * gridDim.x, pa.error, pa.spin_limit);
             if (__hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -399,15 +487,23 @@ hipError_t launch_update_cell(const ExtView<R> &m, int block, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <class R> size_t step_lds(int scheme, int block) {
+    return scheme == 0 ? tile_lds_bytes(block, (int)sizeof(R)) : 0;
+}
+
+// Kernel pointer of a (rare, scheme) instantiation.
+#define HYDRO_PICK(KERNEL, R, VAR, rare, scheme)                                                  \
+    ((rare) ? ((scheme) == 0 ? (const void *)KERNEL<R, VAR, true, 0> : (const void *)KERNEL<R, VAR, true, 1>) \
+            : ((scheme) == 0 ? (const void *)KERNEL<R, VAR, false, 0> : (const void *)KERNEL<R, VAR, false, 1>))
+
 template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
-    if (f.occ4)
-        hipLaunchKernelGGL((k_fused_step_o4<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, kt, jt, remap);
-    else
-        hipLaunchKernelGGL((k_fused_step<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, kt, jt, remap);
-    return hipGetLastError();
+    FusedView<R> a = f;
+    void *args[] = {&a, &kt, &jt, &remap};
+    return hipLaunchKernel(HYDRO_PICK(k_fused_step, R, VAR, f.rare, f.scheme), dim3(div_up(f.C, block)),
+                           dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
 template <class R, int VAR>
@@ -415,13 +511,10 @@ hipError_t launch_fused_step_tick(const FusedView<R> &f, const int64_t *tick, in
                                   int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
-    if (f.occ4)
-        hipLaunchKernelGGL((k_fused_step_tick_o4<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, tick,
-                           offset, spd, remap);
-    else
-        hipLaunchKernelGGL((k_fused_step_tick<R, VAR>), dim3(div_up(f.C, block)), dim3(block), 0, s, f, tick,
-                           offset, spd, remap);
-    return hipGetLastError();
+    FusedView<R> a = f;
+    void *args[] = {&a, &tick, &offset, &spd, &remap};
+    return hipLaunchKernel(HYDRO_PICK(k_fused_step_tick, R, VAR, f.rare, f.scheme), dim3(div_up(f.C, block)),
+                           dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
 hipError_t launch_tick_advance(int64_t *tick, int by, hipStream_t s) {
@@ -430,13 +523,14 @@ hipError_t launch_tick_advance(int64_t *tick, int by, hipStream_t s) {
 }
 
 template <class R, int VAR>
-hipError_t persistent_max_grid(int block, int *grid) {
+hipError_t persistent_max_grid(int block, bool rare, int scheme, int *grid) {
     int dev = 0, per_cu = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_persistent<R, VAR>, block, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, HYDRO_PICK(k_persistent, R, VAR, rare, scheme), block,
+                                                     step_lds<R>(scheme, block));
     if (e != hipSuccess) return e;
     *grid = per_cu * cus;
     return hipSuccess;
@@ -448,7 +542,8 @@ hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, in
     FusedView<R> a = f;
     PersistentArgs p = pa;
     void *args[] = {&a, &p};
-    return hipLaunchCooperativeKernel((const void *)k_persistent<R, VAR>, dim3(grid), dim3(block), args, 0, s);
+    return hipLaunchCooperativeKernel(HYDRO_PICK(k_persistent, R, VAR, f.rare != 0, f.scheme), dim3(grid),
+                                      dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
 template <class R>
@@ -477,8 +572,10 @@ template <class R, int VAR>
 hipError_t launch_batch_step(const BatchView<R> &b, int step, int block, int remap, hipStream_t s) {
     if (b.ntiles == 0) return hipSuccess;
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_batch_step<R, VAR>), dim3(b.ntiles), dim3(block), 0, s, b, step, remap);
-    return hipGetLastError();
+    BatchView<R> bb = b;
+    void *args[] = {&bb, &step, &remap};
+    return hipLaunchKernel(HYDRO_PICK(k_batch_step, R, VAR, b.rare, b.scheme), dim3(b.ntiles), dim3(block), args,
+                           step_lds<R>(b.scheme, block), s);
 }
 
 template <class R, int VAR>
@@ -486,18 +583,21 @@ hipError_t launch_batch_step_tick(const BatchView<R> &b, const int64_t *tick, in
                                   hipStream_t s) {
     if (b.ntiles == 0) return hipSuccess;
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_batch_step_tick<R, VAR>), dim3(b.ntiles), dim3(block), 0, s, b, tick, offset, remap);
-    return hipGetLastError();
+    BatchView<R> bb = b;
+    void *args[] = {&bb, &tick, &offset, &remap};
+    return hipLaunchKernel(HYDRO_PICK(k_batch_step_tick, R, VAR, b.rare, b.scheme), dim3(b.ntiles), dim3(block),
+                           args, step_lds<R>(b.scheme, block), s);
 }
 
 template <class R, int VAR>
-hipError_t batch_persistent_max_grid(int block, int *grid) {
+hipError_t batch_persistent_max_grid(int block, bool rare, int scheme, int *grid) {
     int dev = 0, per_cu = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_batch_persistent<R, VAR>, block, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, HYDRO_PICK(k_batch_persistent, R, VAR, rare, scheme),
+                                                     block, step_lds<R>(scheme, block));
     if (e != hipSuccess) return e;
     *grid = per_cu * cus;
     return hipSuccess;
@@ -510,7 +610,8 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
     BatchView<R> bb = b;
     PersistentArgs p = pa;
     void *args[] = {&bb, &p};
-    return hipLaunchCooperativeKernel((const void *)k_batch_persistent<R, VAR>, dim3(grid), dim3(block), args, 0, s);
+    return hipLaunchCooperativeKernel(HYDRO_PICK(k_batch_persistent, R, VAR, b.rare != 0, b.scheme), dim3(grid),
+                                      dim3(block), args, step_lds<R>(b.scheme, block), s);
 }
 
 #define HYDRO_INSTANTIATE(R, VAR)                                                                        \
@@ -521,13 +622,13 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
                                                        int, int, hipStream_t);                          \
     template hipError_t launch_persistent<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, int, \
                                                   hipStream_t);                                         \
-    template hipError_t persistent_max_grid<R, VAR>(int, int *);                                       \
+    template hipError_t persistent_max_grid<R, VAR>(int, bool, int, int *);                                       \
     template hipError_t launch_batch_step<R, VAR>(const BatchView<R> &, int, int, int, hipStream_t);   \
     template hipError_t launch_batch_step_tick<R, VAR>(const BatchView<R> &, const int64_t *, int, int, int, \
                                                        hipStream_t);                                    \
     template hipError_t launch_batch_persistent<R, VAR>(const BatchView<R> &, const PersistentArgs &, int, int, \
                                                         hipStream_t);                                   \
-    template hipError_t batch_persistent_max_grid<R, VAR>(int, int *);
+    template hipError_t batch_persistent_max_grid<R, VAR>(int, bool, int, int *);
 
 HYDRO_INSTANTIATE(double, 0)
 HYDRO_INSTANTIATE(double, 1)

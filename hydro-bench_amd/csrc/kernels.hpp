@@ -29,7 +29,8 @@ template <class R> struct alignas(4 * sizeof(R)) State {
 template <class R> struct FusedView {
     int C;
     int materialize;           // also write the caller's FLUX0..3 every step
-    int occ4;                  // launch the 128-VGPR variant (4 waves per SIMD)
+    int scheme;                // hydro_scheme: 0 tiles (edge tasks in LDS), 1 one thread per cell
+    int rare;                  // mesh has KLAS 3/5/6/7 or unknown codes: launch the full variant
     Scalars<R> S;
     // the caller's arrays, read only by boundary slots, KLAS 7 rewrites and
     // materialised fluxes; kept behind a device pointer so the common path
@@ -48,6 +49,11 @@ template <class R> struct FusedView {
     const R *COSF, *SINF, *SIDE;    // [4][C]
     const R *SLCOS, *SLSIN;    // [4][C] or null when SIDE*COSF / SIDE*SINF reproduce them
     const int32_t *orig;       // [C]
+    // TILES scheme: tile t (cells [t*block, (t+1)*block)) owns the tasks
+    // [task_off[t], task_off[t+1]).  A task is the slot 4*p + j in task_a
+    // and, for an edge inside the tile, the opposite (mirror) slot in
+    // task_b (-1 otherwise); task_a then holds the primary slot.
+    const int32_t *task_off, *task_a, *task_b;
     // recording for the adjoint (null when not recording): the step's input
     // state, and its flux sums WH, WU, WV as [3][C]
     State<R> *rec_state;
@@ -56,6 +62,10 @@ template <class R> struct FusedView {
 
 constexpr int kMaxBlock = 1024;
 constexpr int kFusedMaxBlock = 256;   // fp64 fused step needs > 128 VGPRs
+
+// Dynamic LDS of a TILES-scheme workgroup of `block` threads: cell states,
+// beds and the three flux-sum contributions of every slot.
+inline size_t tile_lds_bytes(int block, int rsize) { return (size_t)block * 17 * (size_t)rsize; }
 
 // Launch helpers implemented in kernels.hip (explicit instantiations there).
 template <class R, int VAR>
@@ -80,7 +90,7 @@ struct PersistentArgs {
 template <class R, int VAR>
 hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, int block, int grid, hipStream_t s);
 template <class R, int VAR>
-hipError_t persistent_max_grid(int block, int *grid);
+hipError_t persistent_max_grid(int block, bool rare, int scheme, int *grid);
 
 // external H U V Z -> internal records (new order)
 template <class R>
@@ -103,6 +113,8 @@ template <class R> struct BatchView {
     const int64_t *g0;
     const int32_t *spd;
     int ntiles;
+    int rare;     // some member has KLAS 3/5/6/7 or unknown codes
+    int scheme;   // hydro_scheme of every member
 };
 template <class R, int VAR>
 hipError_t launch_batch_step(const BatchView<R> &b, int step, int block, int remap, hipStream_t s);
@@ -113,7 +125,7 @@ template <class R, int VAR>
 hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &pa, int block, int grid,
                                    hipStream_t s);
 template <class R, int VAR>
-hipError_t batch_persistent_max_grid(int block, int *grid);
+hipError_t batch_persistent_max_grid(int block, bool rare, int scheme, int *grid);
 
 // internal statics <- the caller's ZBC AREA FNC (and ZB1 when stored)
 template <class R>

@@ -21,7 +21,10 @@
 //    most once per lane and skipped by the whole wave when no lane needs it.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstdint>
+
+#include "pow_fixed.hpp"
 
 namespace hydro {
 
@@ -62,6 +65,15 @@ __host__ __device__ inline int kind_of_klas(int klas, bool primary_side) {
     return (klas == 0 && primary_side) ? K_PRIMARY : K_MIRROR;
 }
 
+// The float overloads of the math functions, on the host as on the device
+// (an unqualified fmin(float, float) would otherwise find ::fmin(double,
+// double) in a host build and evaluate the fp32 tree's expressions in double).
+using std::copysign;
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::sqrt;
+
 template <class R> struct Consts {
     static constexpr R C0 = (R)1.33;
     static constexpr R C1 = (R)1.7;
@@ -82,12 +94,9 @@ template <> struct SqrtR<double> {
     __host__ __device__ static __forceinline__ double f(double x) { return sqrt(x); }
 };
 template <class R, class X> __host__ __device__ __forceinline__ R sqrt_r(X x) { return SqrtR<R>::f(x); }
-// pow evaluated in double (a float argument is promoted, as in the fp32 tree);
-// the dual number's overload is found by argument-dependent lookup.
-__host__ __device__ __forceinline__ double pow_d(double x, double y) { return pow(x, y); }
-// The same pow, kept out of line: used on the rare branches (wet/dry edges,
-// boundary handlers) so their expansions do not bloat the common path.
-__host__ __device__ __attribute__((noinline)) inline double pow_rare(double x, double y) { return pow(x, y); }
+// pow_d / pow_rare (pow in double with the timestep's fixed exponents) come
+// from pow_fixed.hpp; the dual number's overloads are found by argument-
+// dependent lookup.
 
 template <class R> struct Flux { R f0, f1, f2, f3; };
 
@@ -141,11 +150,16 @@ __host__ __device__ __forceinline__ void osher(R H_pre, R QL0, R QL1, R QL2, R Q
     int K2 = (CA < UA) ? 1 : (UA >= 0.0 && UA < CA) ? 2 : (UA >= -CA && UA < 0.0) ? 3 : 4;
     const bool rsub = (QR1 >= -CR);
     int K1 = (QL1 < CL && rsub) ? 1 : (QL1 >= CL && rsub) ? 2 : (QL1 < CL && QR1 < -CR) ? 3 : 4;
-    const bool lsup = (K1 == 2 || K1 == 4);   // flux starts from the left state
-    const bool rneg = (K1 == 3 || K1 == 4);   // flux ends at the right state
+    // K1 in {2, 4}: the flux starts from the left state; K1 in {3, 4}: it
+    // ends at the right state.  Written as single bit tests: the gfx950
+    // backend of this toolchain mis-lowers the two-case switch that
+    // (K1 == 2 || K1 == 4) becomes, dropping the K1 == 2 negation of the
+    // left sonic-point term (seen at K1 = 2, K2 = 2 with uniform operands).
+    const bool lsup = (K1 & 1) == 0;
+    const bool rneg = K1 >= 3;
     out[0] = 0; out[1] = 0; out[2] = 0; out[3] = 0;
     if (lsup) qf_acc<R>(out, QL0, QL1, QL2, false);
-    if (lsup ? (K2 != 1) : (K2 == 1)) {          // sonic point of the left curve
+    if ((K2 == 1) != lsup) {                     // sonic point of the left curve
         R US = FIL / 3;
         R HS = US * US / 9.81;
         qf_acc<R>(out, HS, US, QL2, lsup);
@@ -158,7 +172,7 @@ __host__ __device__ __forceinline__ void osher(R H_pre, R QL0, R QL1, R QL2, R Q
         R HA = X * X / 39.24;
         qf_acc<R>(out, HA, UA, left ? QL2 : QR2, false);
     }
-    if (rneg ? (K2 != 4) : (K2 == 4)) {          // sonic point of the right curve
+    if ((K2 == 4) != rneg) {                     // sonic point of the right curve
         R US = FIRm / 3;
         R HS = US * US / 9.81;
         qf_acc<R>(out, HS, US, QR2, rneg);
@@ -205,8 +219,10 @@ template <class R> __host__ __device__ inline R urb_iterate(R QLn, R H_pre, R HB
 // Boundary slot (BOUNDA F:513-568).  `prev` supplies FLUX0/1/3 for codes
 // that do not write them.  Sets *klas7_fired when a KLAS 7 slot turns into
 // an interior slot (F:242).
-template <class R, int VAR>
-__host__ __device__ __attribute__((noinline)) Flux<R> boundary_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
+// RARE = false compiles only the kinds the common meshes use (KLAS 10, 1, 4);
+// the caller guarantees no other boundary kind occurs.
+template <class R, int VAR, bool RARE = true>
+__host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
                                                  R H1, R U1, R V1, R Z1, R ZBp, R ZC, R UC, R VC,
                                                  R QL1, R QL2, R FIL, Flux<R> prev,
                                                  bool *klas7_fired) {
@@ -254,6 +270,7 @@ __host__ __device__ __attribute__((noinline)) Flux<R> boundary_flux(int kind, co
         f.f3 = 4.905 * HB1 * HB1;
     } break;
     case K_B3: {
+        if (!RARE) break;
         R QZH3 = (U1 * bc.COSF + V1 * bc.SINF) * H1 * bc.SIDE;
         QZH3 = fmax(QZH3, (R)0.0);
         const auto *QWr = bc.QW + (size_t)bc.cell * (size_t)bc.NHQ;
@@ -279,12 +296,14 @@ __host__ __device__ __attribute__((noinline)) Flux<R> boundary_flux(int kind, co
         f.f3 = 4.905 * H1 * H1;
         break;
     case K_B5: {
+        if (!RARE) break;
         R q = fmax(QL1, (R)0.0);
         f.f0 = H1 * q;
         f.f1 = f.f0 * q;
         f.f3 = 4.905 * H1 * H1 * (1.0 - S.JL) * (1.0 - S.JL);
     } break;
     case K_B6: {
+        if (!RARE) break;
         const R TOP = (R)bc.BF[bc.cell];
         const R cs = bc.COSF, sn = bc.SINF;
         if (Z1 <= TOP && ZC <= TOP) {
@@ -326,6 +345,7 @@ __host__ __device__ __attribute__((noinline)) Flux<R> boundary_flux(int kind, co
         }
     } break;
     case K_B7: {
+        if (!RARE) break;
         const R TOP = (R)bc.BF[bc.cell];
         const R HB = 0;   // the reference reads an uninitialised HB here
         if (Z1 > TOP || ZC > TOP) {
@@ -347,92 +367,70 @@ __host__ __device__ __attribute__((noinline)) Flux<R> boundary_flux(int kind, co
 }
 
 // ---------------------------------------------------------------------------
-// Flux of one non-padding slot (CalculateFluxKernel body F:47-202).
-//   own:  H1 U1 V1 Z1, ZB1 (own ZB1 array value), ZBp (own bed)
-//   nbr:  Hn Un Vn Zn ZBn, valid when has_nbr (NAC != 0)
-template <class R, int VAR>
-__host__ __device__ inline Flux<R> slot_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
-                                             R H1, R U1, R V1, R Z1, R ZB1, R ZBp,
-                                             bool has_nbr, R Hn, R Un, R Vn, R Zn, R ZBn,
-                                             R COSJ, R SINJ, Flux<R> prev, bool *klas7_fired) {
+// Wet/dry ladder of an interior slot (F:113-147), seen from the own cell.
+// Returns the branch: 0 both dry, 1 ZI<=BC, 2 ZC<=ZBp, 3/4 H1<=HM2 (inflow /
+// outflow), 5/6 HC<=HM2, 7 both wet (Riemann problem, F:149-200).  For
+// branches 0..6 *f receives the flux.  At most one pow(., 1.5) and one sqrt
+// per slot, so their expansions appear once in the code.
+template <class R>
+__host__ __device__ __forceinline__ int ladder(const Scalars<R> &S, R H1, R ZI, R ZBp, R HC, R BC, R ZC, R UC,
+                                               R VC, R QL1, R QL2, R COSJ, R SINJ, Flux<R> *f) {
     using K = Consts<R>;
-    const R QL1 = U1 * COSJ + V1 * SINJ;
-    const R QL2 = V1 * COSJ - U1 * SINJ;
-    const R ZI = fmax(Z1, ZB1);
-    R HC = 0, BC = 0, ZC = 0, UC = 0, VC = 0;
-    if (has_nbr) {
-        HC = fmax(Hn, S.HM1);
-        BC = ZBn;
-        ZC = fmax(BC, Zn);
-        UC = Un;
-        VC = Vn;
+    int br;
+    if (H1 <= S.HM1 && HC <= S.HM1) br = 0;
+    else if (ZI <= BC) br = 1;
+    else if (ZC <= ZBp) br = 2;
+    else if (H1 <= S.HM2) br = (ZC > ZI) ? 3 : 4;
+    else if (HC <= S.HM2) br = (ZI > ZC) ? 5 : 6;
+    else br = 7;
+    if (br == 7) return br;
+    // Branch sets as bit masks over br (no multi-case switch: see osher):
+    // pow(., 1.5) for {1, 2, 4, 6} of HC in {1, 6}, sqrt for {3, 5}.
+    const bool use_p = (0x56 >> br) & 1, p_hc = (0x42 >> br) & 1, use_sq = (0x28 >> br) & 1;
+    R p15 = 0;
+    if (use_p) p15 = pow_rare(p_hc ? HC : H1, 1.5);
+    R DH = 0, UN = 0;
+    if (use_sq) {
+        DH = fmax(br == 3 ? ZC - ZBp : ZI - BC, S.HM1);
+        const R sq = sqrt_r<R>(DH);
+        UN = (br == 3) ? -K::C1 * sq : K::C1 * sq;
     }
-    Flux<R> f;
-    if (kind >= K_B10) {
-        const R CL = sqrt_r<R>(9.81 * H1);
-        const R FIL = QL1 + 2 * CL;
-        return boundary_flux<R, VAR>(kind, S, bc, H1, U1, V1, Z1, ZBp, ZC, UC, VC, QL1, QL2, FIL,
-                                     prev, klas7_fired);
-    }
-    if (H1 <= S.HM1 && HC <= S.HM1) {
-        f.f0 = 0; f.f1 = 0; f.f2 = 0; f.f3 = 0;
-        return f;
-    }
-    if (ZI <= BC) {
-        f.f0 = -K::C1 * pow_rare(HC, 1.5);
-        f.f1 = H1 * QL1 * fabs(QL1);
-        f.f2 = 0;
-        f.f3 = 4.905 * H1 * H1;
-        return f;
-    }
-    if (ZC <= ZBp) {
-        f.f0 = K::C1 * pow_rare(H1, 1.5);
-        f.f1 = H1 * fabs(QL1) * QL1;
-        f.f2 = H1 * fabs(QL1) * QL2;
-        f.f3 = 0;
-        return f;
-    }
-    if (H1 <= S.HM2) {
-        if (ZC > ZI) {
-            R DH = fmax(ZC - ZBp, S.HM1);
-            R UN = -K::C1 * sqrt_r<R>(DH);
-            f.f0 = DH * UN;
-            f.f1 = f.f0 * UN;
-            f.f2 = f.f0 * (VC * COSJ - UC * SINJ);
-        } else {
-            f.f0 = K::C1 * pow_rare(H1, 1.5);
-            f.f1 = 0;
-            f.f2 = 0;
-        }
-        f.f3 = 4.905 * H1 * H1;
-        return f;
-    }
-    if (HC <= S.HM2) {
-        if (ZI > ZC) {
-            R DH = fmax(ZI - BC, S.HM1);
-            R UN = K::C1 * sqrt_r<R>(DH);
-            R HC1 = ZC - ZBp;
-            f.f0 = DH * UN;
-            f.f1 = f.f0 * UN;
-            f.f2 = f.f0 * QL2;
-            f.f3 = 4.905 * HC1 * HC1;
-        } else {
-            f.f0 = -K::C1 * pow_rare(HC, 1.5);
-            f.f1 = H1 * QL1 * QL1;
-            f.f2 = 0;
-            f.f3 = 4.905 * H1 * H1;
-        }
-        return f;
-    }
-    // Both sides wet: one Riemann solve on (left a, right b).  Primary side:
-    // a = own, b = neighbour, normal (COSJ, SINJ).  Mirror side: a = neighbour,
-    // b = own, normal negated -- the neighbour's own primary problem.
-    const bool prim = (kind == K_PRIMARY);
-    const R c = prim ? COSJ : -COSJ;
-    const R s = prim ? SINJ : -SINJ;
-    const R Ha = prim ? H1 : Hn, Ua = prim ? U1 : Un, Va = prim ? V1 : Vn, ZBa = prim ? ZBp : ZBn;
-    const R Hb = prim ? Hn : H1, Ub = prim ? Un : U1, Vb = prim ? Vn : V1;
-    const R ZBb = prim ? ZBn : ZBp, Zb = prim ? Zn : Z1;
+    // every branch's expressions, in the reference's evaluation order,
+    // then selected (F:121-147)
+    const R f0_dh = DH * UN;                    // 3, 5
+    const R f0_pn = -K::C1 * p15;               // 1, 6
+    const R f0_pp = K::C1 * p15;                // 2, 4
+    const R f1_1 = H1 * QL1 * fabs(QL1);        // 1
+    const R f1_2 = H1 * fabs(QL1) * QL1;        // 2
+    const R f1_dh = f0_dh * UN;                 // 3, 5
+    const R f1_6 = H1 * QL1 * QL1;              // 6
+    const R f2_2 = H1 * fabs(QL1) * QL2;        // 2
+    const R f2_3 = f0_dh * (VC * COSJ - UC * SINJ);
+    const R f2_5 = f0_dh * QL2;
+    const R HC1 = ZC - ZBp;
+    const R f3_h = 4.905 * H1 * H1;             // 1, 3, 4, 6
+    const R f3_5 = 4.905 * HC1 * HC1;           // 5
+    f->f0 = br == 0 ? (R)0 : use_sq ? f0_dh : p_hc ? f0_pn : f0_pp;
+    f->f1 = br == 1 ? f1_1 : br == 2 ? f1_2 : use_sq ? f1_dh : br == 6 ? f1_6 : (R)0;
+    f->f2 = br == 2 ? f2_2 : br == 3 ? f2_3 : br == 5 ? f2_5 : (R)0;
+    f->f3 = (br == 0 || br == 2) ? (R)0 : br == 5 ? f3_5 : f3_h;
+    return br;
+}
+
+// Riemann problem of an interior edge in its primary orientation: left
+// state a (the primary cell), right state b, normal (c, s) of a's slot
+// (F:151-164).  Both sides of the edge use the same FO and f1 (F:165-200
+// solves the neighbour's primary problem on the mirror side).
+template <class R> struct EdgeSol {
+    R FO0, FO2, FO3;
+    R f1;    // FO[1] plus the depth-ratio correction
+    R ZBa;   // bed of the primary cell (mirror side's F3)
+};
+
+template <class R>
+__host__ __device__ __forceinline__ EdgeSol<R> edge_solve(const Scalars<R> &S, R Ha, R Ua, R Va, R ZBa, R Hb, R Ub,
+                                                          R Vb, R Zb, R ZBb, R c, R s) {
+    using K = Consts<R>;
     const R qa1 = Ua * c + Va * s;
     const R qa2 = Va * c - Ua * s;
     const R CLa = sqrt_r<R>(9.81 * Ha);
@@ -447,19 +445,93 @@ __host__ __device__ inline Flux<R> slot_flux(int kind, const Scalars<R> &S, cons
     const R QR2 = Vb * c - Ub * s;
     R FO[4];
     osher<R>(Ha, Ha, qa1, qa2, QR0, QR1, QR2, FILa, FO);
-    f.f1 = FO[1] + (1 - fmin(HCb / QR0, (R)1.5)) * HCb * UR * UR / 2;
-    f.f2 = FO[2];
+    EdgeSol<R> e;
+    e.FO0 = FO[0];
+    e.FO2 = FO[2];
+    e.FO3 = FO[3];
+    e.f1 = FO[1] + (1 - fmin(HCb / QR0, (R)1.5)) * HCb * UR * UR / 2;
+    e.ZBa = ZBa;
+    return e;
+}
+
+// Flux of the primary (prim) or mirror side of a solved edge; ZBp is the
+// own cell's bed.
+template <class R>
+__host__ __device__ __forceinline__ Flux<R> edge_side(const EdgeSol<R> &e, bool prim, R ZBp) {
+    Flux<R> f;
+    f.f1 = e.f1;
+    f.f2 = e.FO2;
     if (prim) {
-        f.f0 = FO[0];
-        f.f3 = FO[3];
+        f.f0 = e.FO0;
+        f.f3 = e.FO3;
     } else {
-        f.f0 = -FO[0];
-        R ZA = sqrt_r<R>(FO[3] / 4.905) + ZBa;
+        f.f0 = -e.FO0;
+        R ZA = sqrt_r<R>(e.FO3 / 4.905) + e.ZBa;
         R HC3 = 0;
         if (ZA > ZBp) HC3 = ZA - ZBp;
         f.f3 = 4.905 * HC3 * HC3;
     }
     return f;
+}
+
+// Flux of one non-padding slot (CalculateFluxKernel body F:47-202).
+//   own:  H1 U1 V1 Z1, ZB1 (own ZB1 array value), ZBp (own bed)
+//   nbr:  Hn Un Vn Zn ZBn, valid when has_nbr (NAC != 0)
+template <class R, int VAR, bool RARE = true>
+__host__ __device__ inline Flux<R> slot_flux(int kind, const Scalars<R> &S, const BcCtx<R> &bc,
+                                             R H1, R U1, R V1, R Z1, R ZB1, R ZBp,
+                                             bool has_nbr, R Hn, R Un, R Vn, R Zn, R ZBn,
+                                             R COSJ, R SINJ, Flux<R> prev, bool *klas7_fired) {
+    const R QL1 = U1 * COSJ + V1 * SINJ;
+    const R QL2 = V1 * COSJ - U1 * SINJ;
+    const R ZI = fmax(Z1, ZB1);
+    R HC = 0, BC = 0, ZC = 0, UC = 0, VC = 0;
+    if (has_nbr) {
+        HC = fmax(Hn, S.HM1);
+        BC = ZBn;
+        ZC = fmax(BC, Zn);
+        UC = Un;
+        VC = Vn;
+    }
+    if (kind >= K_B10) {
+        const R CL = sqrt_r<R>(9.81 * H1);
+        const R FIL = QL1 + 2 * CL;
+        return boundary_flux<R, VAR, RARE>(kind, S, bc, H1, U1, V1, Z1, ZBp, ZC, UC, VC, QL1, QL2, FIL,
+                                           prev, klas7_fired);
+    }
+    Flux<R> f;
+    if (ladder<R>(S, H1, ZI, ZBp, HC, BC, ZC, UC, VC, QL1, QL2, COSJ, SINJ, &f) < 7) return f;
+    // Both sides wet: one Riemann solve on (left a, right b).  Primary side:
+    // a = own, b = neighbour, normal (COSJ, SINJ).  Mirror side: a = neighbour,
+    // b = own, normal negated -- the neighbour's own primary problem.
+    const bool prim = (kind == K_PRIMARY);
+    const R c = prim ? COSJ : -COSJ;
+    const R s = prim ? SINJ : -SINJ;
+    const EdgeSol<R> e = edge_solve<R>(S, prim ? H1 : Hn, prim ? U1 : Un, prim ? V1 : Vn, prim ? ZBp : ZBn,
+                                       prim ? Hn : H1, prim ? Un : U1, prim ? Vn : V1, prim ? Zn : Z1,
+                                       prim ? ZBn : ZBp, c, s);
+    return edge_side<R>(e, prim, ZBp);
+}
+
+// Both slots of an interior edge whose sides are the primary cell P and the
+// mirror cell M (normals exactly opposite), each through its own ladder and
+// one shared Riemann solve: the same values as slot_flux on each side.
+template <class R>
+__host__ __device__ __forceinline__ void pair_flux(const Scalars<R> &S, R HP, R UP, R VP, R ZP, R ZB1P, R ZBP,
+                                                   R HM, R UM, R VM, R ZM, R ZB1M, R ZBM, R c, R s,
+                                                   Flux<R> *fP, Flux<R> *fM) {
+    const R cm = -c, sm = -s;
+    const R QL1P = UP * c + VP * s, QL2P = VP * c - UP * s;
+    const R QL1M = UM * cm + VM * sm, QL2M = VM * cm - UM * sm;
+    const int bP = ladder<R>(S, HP, fmax(ZP, ZB1P), ZBP, fmax(HM, S.HM1), ZBM, fmax(ZBM, ZM), UM, VM, QL1P, QL2P,
+                             c, s, fP);
+    const int bM = ladder<R>(S, HM, fmax(ZM, ZB1M), ZBM, fmax(HP, S.HM1), ZBP, fmax(ZBP, ZP), UP, VP, QL1M, QL2M,
+                             cm, sm, fM);
+    if (bP == 7 || bM == 7) {
+        const EdgeSol<R> e = edge_solve<R>(S, HP, UP, VP, ZBP, HM, UM, VM, ZM, ZBM, c, s);
+        if (bP == 7) *fP = edge_side<R>(e, true, ZBP);
+        if (bM == 7) *fM = edge_side<R>(e, false, ZBM);
+    }
 }
 
 // Sign-exact stand-in for the flux of a triangle's padding slot (NAC 0,
@@ -549,15 +621,23 @@ __host__ __device__ __forceinline__ CellOut<R> update_math(const Scalars<R> &S, 
     return o;
 }
 
-// Accumulate one slot into the cell sums in the reference's order (F:732-742).
+// One slot's terms of the cell sums WH, WU, WV (F:732-742).
+template <class R> struct Contrib { R h, u, v; };
+template <class R>
+__host__ __device__ __forceinline__ Contrib<R> contrib(const Flux<R> &f, R SL, R SLCA, R SLSA) {
+    const R FLR_1 = f.f1 + f.f3;
+    const R FLR_2 = f.f2;
+    return Contrib<R>{SL * f.f0, SLCA * FLR_1 - SLSA * FLR_2, SLSA * FLR_1 + SLCA * FLR_2};
+}
+
+// Accumulate one slot into the cell sums in the reference's order.
 template <class R>
 __host__ __device__ __forceinline__ void accumulate(R &WH, R &WU, R &WV, const Flux<R> &f, R SL, R SLCA,
                                                     R SLSA) {
-    const R FLR_1 = f.f1 + f.f3;
-    const R FLR_2 = f.f2;
-    WH += SL * f.f0;
-    WU += SLCA * FLR_1 - SLSA * FLR_2;
-    WV += SLSA * FLR_1 + SLCA * FLR_2;
+    const Contrib<R> c = contrib<R>(f, SL, SLCA, SLSA);
+    WH += c.h;
+    WU += c.u;
+    WV += c.v;
 }
 
 }  // namespace hydro

@@ -102,6 +102,55 @@ std::vector<int32_t> rcm_order(int C, const int32_t *NAC) {
     return order;
 }
 
+std::vector<int32_t> tile_order(int C, const int32_t *NAC, int B, const std::vector<int32_t> &along) {
+    Csr g = cell_graph(C, NAC);
+    std::vector<int32_t> rank(C);
+    for (int i = 0; i < C; i++) rank[along[i]] = i;
+    std::vector<char> taken(C, 0);
+    std::vector<int32_t> order;
+    order.reserve(C);
+    std::vector<int32_t> queue;
+    size_t next = 0;   // first position of `along` that may still be free
+    int hint = -1;     // seed proposed by the previous tile's frontier
+    auto global_seed = [&]() {
+        while (next < (size_t)C && taken[along[next]]) next++;
+        return next < (size_t)C ? along[next] : -1;
+    };
+    while ((int)order.size() < C) {
+        const size_t t0 = order.size();
+        const size_t want = std::min<size_t>((size_t)B, (size_t)C - t0);
+        int seed = (hint >= 0 && !taken[hint]) ? hint : global_seed();
+        while (order.size() - t0 < want) {
+            if (seed < 0) seed = global_seed();
+            queue.clear();
+            queue.push_back(seed);
+            taken[seed] = 1;
+            order.push_back(seed);
+            seed = -1;
+            for (size_t h = 0; h < queue.size() && order.size() - t0 < want; h++) {
+                const int v = queue[h];
+                for (int k = g.start[v]; k < g.start[v + 1] && order.size() - t0 < want; k++) {
+                    const int w = g.adj[k];
+                    if (taken[w]) continue;
+                    taken[w] = 1;
+                    order.push_back(w);
+                    queue.push_back(w);
+                }
+            }
+        }
+        // next seed: the free neighbour of this tile earliest in `along`
+        hint = -1;
+        for (size_t i = t0; i < order.size(); i++) {
+            const int v = order[i];
+            for (int k = g.start[v]; k < g.start[v + 1]; k++) {
+                const int w = g.adj[k];
+                if (!taken[w] && (hint < 0 || rank[w] < rank[hint])) hint = w;
+            }
+        }
+    }
+    return order;
+}
+
 namespace {
 // Hilbert index of (x, y) in a 2^order grid (iterative rotate-and-flip).
 uint64_t hilbert_d(uint32_t n, uint32_t x, uint32_t y) {

@@ -10,6 +10,15 @@ namespace hydro {
 // neighbour array NAC[4*C] (0 = none).  Returns perm with perm[new] = old.
 std::vector<int32_t> rcm_order(int C, const int32_t *NAC);
 
+// Cells grouped into tiles of B consecutive positions, each grown breadth-
+// first from a seed so that a tile is a compact patch holding most of its
+// cells' neighbours.  The next tile starts from the free neighbour of the
+// previous tile that comes first in `along` (a permutation, e.g. RCM), else
+// from the first free cell of `along`; a tile whose growth runs out of
+// connected cells continues from such a seed, so every tile but the last
+// has exactly B cells.  perm[new] = old.
+std::vector<int32_t> tile_order(int C, const int32_t *NAC, int B, const std::vector<int32_t> &along);
+
 // Hilbert-curve order of cell centroids (xc, yc), perm[new] = old.
 std::vector<int32_t> hilbert_order(int C, const double *xc, const double *yc);
 

@@ -31,7 +31,8 @@ MODES = {
     "graph_two_kernel": MODE_GRAPH_TWO_KERNEL,
 }
 # hydro_reorder
-REORDER_NONE, REORDER_RCM, REORDER_HILBERT = range(3)
+REORDER_NONE, REORDER_RCM, REORDER_HILBERT, REORDER_TILES = range(4)
+SCHEME_TILES, SCHEME_CELL = range(2)
 
 
 class HydroError(RuntimeError):
@@ -66,13 +67,13 @@ class Tune(ctypes.Structure):
     _fields_ = [
         ("block", c_int32), ("reorder", c_int32), ("xcd_remap", c_int32), ("graph_chunk", c_int32),
         ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("waves_per_simd", c_int32),
-        ("reserved", c_int32),
+        ("scheme", c_int32),
     ]
 
 
 def default_tune(**kw) -> Tune:
-    t = Tune(block=256, reorder=REORDER_RCM, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
-             materialize_flux=0, waves_per_simd=0)
+    t = Tune(block=256, reorder=REORDER_TILES, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
+             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES)
     for k, v in kw.items():
         setattr(t, k, v)
     return t
@@ -98,6 +99,7 @@ SIGNATURES = {
     "hydro_precalculate": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_double] + [c_void_p] * 9),
     "hydro_rcm_order": (c_int, [c_int, c_void_p, c_void_p]),
+    "hydro_tile_order": (c_int, [c_int, c_void_p, c_int, c_void_p]),
     # adjoint / tangent-linear support
     "hydro_adj_sizes": (c_int, [c_void_p, POINTER(ctypes.c_size_t), POINTER(ctypes.c_size_t),
                                 POINTER(ctypes.c_size_t)]),

@@ -25,6 +25,7 @@ from __future__ import annotations
 import ctypes
 import dataclasses
 import math
+import os
 from typing import Dict, Optional
 
 import numpy as np
@@ -350,14 +351,16 @@ def _klas_series(raw, Lx, inflow_mask, outflow_mask, q_of_t, z_out, n_extra_q0, 
     return KLAS, q_series, z_series
 
 
-def make_channel_mesh(seed=1001, cells=6675, steps=100, DT=1.0, MDT=60, dtype=np.float64, lib=None):
+def make_channel_mesh(seed=1001, cells=6675, steps=100, DT=1.0, MDT=60, dtype=np.float64, lib=None, rotate=0.0):
     """Plain forward, F1-style: a 3 km x 1 km channel of mixed triangles and quads.
 
     Bed: 1e-3 planar slope plus 5 Gaussian bumps of 0.5 m; initial depth
     1.0 +- 0.05 m at rest; Manning n uniform in [0.02, 0.035]; walls (KLAS 4)
     on the sides, an upstream discharge set (KLAS 10) carrying
     50 +- 20 m^3/s with a 3600 s period, a fixed downstream level (KLAS 1)
-    and 8 more KLAS 10 edges with QT = 0.
+    and 8 more KLAS 10 edges with QT = 0.  `rotate` (radians) turns the
+    finished geometry about the origin, so that no boundary normal has a
+    zero component.
     """
     rng = np.random.default_rng(seed)
     Lx, Ly = 3000.0, 1000.0
@@ -378,11 +381,17 @@ def make_channel_mesh(seed=1001, cells=6675, steps=100, DT=1.0, MDT=60, dtype=np
     KLAS, qs, zs = _klas_series(raw, Lx, inflow, outflow, q, z_out, 8, rng, NDAYS, MDT)
     raw.update(ZBC=bed, Zinit=bed + h0, n=n, KLAS=KLAS, q_series=qs, z_series=zs, NDAYS=NDAYS, MDT=MDT, DT=DT,
                HM1=0.001, HM2=0.01, JL=0.0)
+    if rotate:
+        c, s = math.cos(rotate), math.sin(rotate)
+        for kx, ky in (("XP", "YP"), ("xc", "yc")):
+            x, y = raw[kx].copy(), raw[ky].copy()
+            raw[kx], raw[ky] = c * x - s * y, s * x + c * y
     return _finish(f"channel_{cells}", raw, dtype, lib)
 
 
 def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dtype=np.float64, lib=None,
-                         Lx=20000.0, Ly=8000.0, q_peak=400.0):
+                         Lx=20000.0, Ly=8000.0, q_peak=400.0, n_bumps=12, n_channel=0.03, n_plain=0.05,
+                         slope=2e-4):
     """Plain forward, F2-style: a 20 km x 8 km floodplain with a meandering
     main channel carved 3 m into the bed; depths 0-4 m with wet/dry fronts;
     Manning n 0.03 in the channel and 0.05 on the floodplain.
@@ -394,9 +403,9 @@ def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dty
     rng = np.random.default_rng(seed)
     raw = _raw_mesh(Lx, Ly, cells, 0.0015, rng, aspect=Lx / Ly)
     xc, yc = raw["xc"], raw["yc"]
-    nb = 12
+    nb = int(n_bumps)
     centers = np.stack([rng.uniform(0.05 * Lx, 0.95 * Lx, nb), rng.uniform(0.05 * Ly, 0.95 * Ly, nb)], 1)
-    plain = 10.0 + 2e-4 * (Lx - xc)
+    plain = 10.0 + slope * (Lx - xc)
     bumps = _gaussians(xc, yc, centers, rng.uniform(0.3, 1.5, nb), rng.uniform(0.03 * Lx, 0.06 * Lx, nb) * 0.5)
     phase = rng.uniform(0, 2 * math.pi)
     wav = 0.3 * Lx
@@ -407,7 +416,7 @@ def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dty
     in_channel = dist < hw
     bed = plain + bumps * (~in_channel) - carve
     level = plain + 0.8
-    n = np.where(in_channel, 0.03, 0.05)
+    n = np.where(in_channel, n_channel, n_plain)
     spd = int(MDT / DT)
     NDAYS = math.ceil(steps / max(spd - 1, 1)) + 1
     eps = 1e-6 * Lx
@@ -421,6 +430,42 @@ def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dty
     raw.update(ZBC=bed, Zinit=level, n=n, KLAS=KLAS, q_series=qs, z_series=zs, NDAYS=NDAYS, MDT=MDT, DT=DT,
                HM1=0.001, HM2=0.01, JL=0.0)
     return _finish(f"floodplain_{cells}", raw, dtype, lib)
+
+
+def ensemble_specs(seed=3003, n=128, lo=2000, hi=207234):
+    """Per-member parameters of the batched ensemble: cell counts log-uniform
+    in [lo, hi]; domains scaled to keep the floodplain's cell size; each member
+    its own bed (slope plus 1-8 bumps), Manning pair and inflow peak
+    (10-500 m^3/s)."""
+    rng = np.random.default_rng(seed)
+    cells = np.exp(rng.uniform(math.log(lo), math.log(hi), n)).astype(int)
+    specs = []
+    for i, c in enumerate(cells.tolist()):
+        area = c * (20000.0 * 8000.0 / 207234)
+        Ly = math.sqrt(area / 2.5)
+        specs.append(dict(seed=int(seed * 1000 + i), cells=int(c), Lx=2.5 * Ly, Ly=Ly,
+                          q_peak=float(rng.uniform(10, 500)), n_bumps=int(rng.integers(1, 9)),
+                          n_channel=float(rng.uniform(0.025, 0.035)), n_plain=float(rng.uniform(0.04, 0.06)),
+                          slope=float(rng.uniform(1e-4, 4e-4))))
+    return specs
+
+
+def _ensemble_member(args):
+    spec, steps, dtype = args
+    return make_floodplain_mesh(steps=steps, DT=0.5, dtype=dtype, **spec)
+
+
+def make_ensemble(seed=3003, n=128, lo=2000, hi=207234, steps=500, dtype=np.float32, workers=None):
+    """The members of the batched ensemble, built in parallel processes."""
+    specs = ensemble_specs(seed, n, lo, hi)
+    jobs = [(sp, steps, dtype) for sp in specs]
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or n == 1:
+        return [_ensemble_member(j) for j in jobs]
+    import multiprocessing as mp
+
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_ensemble_member, jobs, chunksize=1)
 
 
 # Workloads (sizes and time steps).  "channel_6675" is the CPU-runnable

@@ -114,7 +114,26 @@ typedef struct hydro_mesh_desc {
 typedef struct hydro_mesh hydro_mesh; /* opaque handle */
 
 /* Tuning of the reordered (FUSED / GRAPH / PERSISTENT) path. */
-typedef enum hydro_reorder { HYDRO_REORDER_NONE = 0, HYDRO_REORDER_RCM = 1, HYDRO_REORDER_HILBERT = 2 } hydro_reorder;
+/* Cell order of the internal layout.  TILES: compact breadth-first tiles of
+ * `block` cells seeded along the RCM order, so most edges of a tile join two
+ * of its own cells (the TILES scheme solves those edges once). */
+typedef enum hydro_reorder {
+    HYDRO_REORDER_NONE = 0,
+    HYDRO_REORDER_RCM = 1,
+    HYDRO_REORDER_HILBERT = 2,
+    HYDRO_REORDER_TILES = 3
+} hydro_reorder;
+
+/* Work split of the FUSED / GRAPH / PERSISTENT step kernels.
+ *  TILES: a workgroup owns a tile of `block` cells; it stages their states
+ *         in LDS, solves each interior edge inside the tile once for both
+ *         slots (other slots one by one), keeps the slot contributions in
+ *         LDS, then each thread sums its cell's four slots in slot order and
+ *         updates the cell.
+ *  CELL:  one thread per cell computes its four slot fluxes (each interior
+ *         edge is solved from both sides) and updates the cell.
+ * Both give the same values bit for bit. */
+typedef enum hydro_scheme { HYDRO_SCHEME_TILES = 0, HYDRO_SCHEME_CELL = 1 } hydro_scheme;
 
 typedef struct hydro_tune {
     int32_t block;          /* threads per workgroup of the fused step (64..256, multiple of 64) */
@@ -125,7 +144,7 @@ typedef struct hydro_tune {
     int32_t materialize_flux; /* 1: FUSED modes also write FLUX0..3 every step */
     int32_t waves_per_simd; /* 0: compiler's choice; 4: fused kernels limited to 128 VGPRs so
                                4 waves fit per SIMD (one resident round at 207K cells) */
-    int32_t reserved;
+    int32_t scheme;         /* hydro_scheme */
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -262,6 +281,11 @@ int hydro_precalculate(int dtype, int CELL, int NOD, const int32_t *NAP, void *X
 /* Reverse Cuthill-McKee order of the cell graph of a host NAC array:
  * perm[new] = old.  The reordered layout of hydro_prepare uses it. */
 int hydro_rcm_order(int CELL, const int32_t *NAC, int32_t *perm);
+
+/* The TILES cell order of hydro_prepare for tiles of `block` cells: compact
+ * breadth-first tiles seeded along the RCM order, every tile but the last
+ * full.  perm[new] = old. */
+int hydro_tile_order(int CELL, const int32_t *NAC, int block, int32_t *perm);
 
 #ifdef __cplusplus
 }
