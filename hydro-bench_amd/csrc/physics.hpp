@@ -387,7 +387,10 @@ __host__ __device__ __forceinline__ int ladder(const Scalars<R> &S, R H1, R ZI, 
     // Branch sets as bit masks over br (no multi-case switch: see osher):
     // pow(., 1.5) for {1, 2, 4, 6} of HC in {1, 6}, sqrt for {3, 5}.
     const bool use_p = (0x56 >> br) & 1, p_hc = (0x42 >> br) & 1, use_sq = (0x28 >> br) & 1;
-    R p15 = 0;
+    // pow in double, kept in double until it meets C1 (F:119-145 multiply
+    // the double result; rounding it to R first would round twice in fp32)
+    using P = decltype(pow_rare(H1, 1.5));
+    P p15 = 0;
     if (use_p) p15 = pow_rare(p_hc ? HC : H1, 1.5);
     R DH = 0, UN = 0;
     if (use_sq) {

@@ -51,3 +51,32 @@ def rel_err(got, ref):
     if d == 0.0:
         return 0.0
     return d / s if s > 0 else np.inf
+
+
+def rare_kinds_mesh(host, breach_fires):
+    """A copy of `host` whose wall slots are recoded round-robin to KLAS 5
+    (open), 6 (weir), 7 (breach), 3 (rating curve), 2 (no handler) and 4,
+    with weir crests inside the water column and a 4-point rating curve;
+    breach crests too when `breach_fires`, else above the water."""
+    h = host.copy()
+    a = h.arrays
+    walls = np.nonzero(a["KLAS"] == 4)[0]
+    codes = np.array([5, 6, 7, 3, 2, 4], dtype=np.int32)
+    a["KLAS"][walls] = codes[np.arange(len(walls)) % len(codes)]
+    cells = walls // 4
+    dt = a["H"].dtype
+    a["BoundaryFeature"][:] = 0
+    a["BoundaryFeature"][cells] = (a["ZBC"][cells] + 0.5 * a["H"][cells]).astype(dt)
+    if not breach_fires:
+        b7 = cells[a["KLAS"][walls] == 7]
+        a["BoundaryFeature"][b7] = (a["ZBC"][b7] + 3.0 * a["H"][b7] + 1.0).astype(dt)
+    nhq = 4
+    C = h.CELL
+    h.NHQ = nhq
+    q = np.array([0.0, 0.05, 0.2, 1.0])
+    dz = np.array([0.2, 0.5, 0.9, 1.6])
+    a["QW"] = np.tile(q, C).astype(dt)
+    a["ZW"] = (np.repeat(a["ZBC"].astype(np.float64), nhq) + np.tile(dz, C)).astype(dt)
+    a["NHQ1"] = np.zeros(C, np.int32)
+    a["NHQ1"][cells[a["KLAS"][walls] == 3]] = nhq
+    return h

@@ -15,37 +15,12 @@ interior ladder (under variant B breaches fire and turn interior).  After 30 ste
 import numpy as np
 import pytest
 
-from conftest import cached_mesh, rel_err
+from conftest import cached_mesh, rare_kinds_mesh, rel_err
 
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("H", "U", "V", "Z", "W")
 FLUXES = ("FLUX0", "FLUX1", "FLUX2", "FLUX3")
-
-
-def _rare_kinds(host, breach_fires):
-    h = host.copy()
-    a = h.arrays
-    walls = np.nonzero(a["KLAS"] == 4)[0]
-    codes = np.array([5, 6, 7, 3, 2, 4], dtype=np.int32)
-    a["KLAS"][walls] = codes[np.arange(len(walls)) % len(codes)]
-    cells = walls // 4
-    dt = a["H"].dtype
-    a["BoundaryFeature"][:] = 0
-    a["BoundaryFeature"][cells] = (a["ZBC"][cells] + 0.5 * a["H"][cells]).astype(dt)
-    if not breach_fires:
-        b7 = cells[a["KLAS"][walls] == 7]
-        a["BoundaryFeature"][b7] = (a["ZBC"][b7] + 3.0 * a["H"][b7] + 1.0).astype(dt)
-    nhq = 4
-    C = h.CELL
-    h.NHQ = nhq
-    q = np.array([0.0, 0.05, 0.2, 1.0])
-    dz = np.array([0.2, 0.5, 0.9, 1.6])
-    a["QW"] = np.tile(q, C).astype(dt)
-    a["ZW"] = (np.repeat(a["ZBC"].astype(np.float64), nhq) + np.tile(dz, C)).astype(dt)
-    a["NHQ1"] = np.zeros(C, np.int32)
-    a["NHQ1"][cells[a["KLAS"][walls] == 3]] = nhq
-    return h
 
 
 def _dm(host, variant, **tune):
@@ -65,7 +40,7 @@ def test_schemes_modes_and_rare_kinds(oracle, dtype, variant):
     # The channel is turned by 0.5 rad: with axis-aligned walls a fired
     # breach meets SIDE*COSF == 0, and 0 * inf is a NaN whose sign (which
     # copysign then reads) no arithmetic standard fixes.
-    host = _rare_kinds(cached_mesh("channel", dtype=dtype, rotate=0.5), breach_fires=(variant == 1))
+    host = rare_kinds_mesh(cached_mesh("channel", dtype=dtype, rotate=0.5), breach_fires=(variant == 1))
     steps = 30
     orc = oracle.OracleRun(host, variant)
     orc.step(steps)
