@@ -11,7 +11,7 @@ Prints ONE JSON line:
   cpu_baseline the oracle (C++ restatement of the reference kernels) timed
               on a bounded sample of the same workload, 1 host thread
 
-Usage: python bench.py [--gpus 1] [--steps 900] [--warmup 20] [--mode persistent]
+Usage: python bench.py [--gpus 1] [--steps 900] [--warmup 20] [--mode fused]
 """
 import argparse
 import json
@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=900)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--mode", default="persistent", choices=["persistent", "graph", "fused", "two_kernel"])
+    ap.add_argument("--mode", default="fused", choices=["persistent", "graph", "fused", "two_kernel"])
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -113,8 +113,8 @@ def main():
     if a.mode == "two_kernel":
         bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C / 2.0
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-    kernel = {"persistent": "k_persistent<double,0>", "graph": "k_fused_step_tick<double,0>",
-              "fused": "k_fused_step<double,0>", "two_kernel": "k_calculate_flux+k_update_cell"}[a.mode]
+    kernel = {"persistent": "k_persistent<double,0,false,0>", "graph": "k_fused_step_tick<double,0,false,0>",
+              "fused": "k_fused_step<double,0,false,0>", "two_kernel": "k_calculate_flux+k_update_cell"}[a.mode]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{a.mode}_{WORKLOAD}_f64.json")
     if os.path.exists(tpath):

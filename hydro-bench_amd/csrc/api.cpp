@@ -107,10 +107,11 @@ struct hydro_mesh {
     int cur = 0;  // state record buffer holding the current state
     bool ext_valid = true, int_valid = false;
     DevBuf st[2], W, ZBC, AREA, FNC, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
-    DevBuf task_off, task_a, task_b;   // TILES scheme task lists
+    DevBuf task_w, task_nb, task_cs, task_sd, cell_pad;   // TILES scheme task entries
+    int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
-    DevBuf tick, barrier, perr, ext_dev;
+    DevBuf tick, barrier, perr, ext_dev;   // tick: StepClock of the GRAPH mode
     bool need_zb1 = false, need_slcs = false;
     hipStream_t cap_stream = nullptr;
     std::map<GraphKey, std::unique_ptr<GraphExec>> graphs;
@@ -176,11 +177,15 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.SLCOS = m->need_slcs ? m->SLCOS.as<R>() : nullptr;
     f.SLSIN = m->need_slcs ? m->SLSIN.as<R>() : nullptr;
     f.orig = m->orig.as<int32_t>();
-    f.task_off = m->task_off.as<int32_t>();
-    f.task_a = m->task_a.as<int32_t>();
-    f.task_b = m->task_b.as<int32_t>();
+    f.task_stride = m->task_stride;
+    f.task_w = m->task_w.as<uint32_t>();
+    f.task_nb = m->task_nb.as<int32_t>();
+    f.task_cs = m->task_cs.as<R>();
+    f.task_sd = m->task_sd.as<R>();
+    f.cell_pad = m->cell_pad.as<uint8_t>();
     f.materialize = m->tune.materialize_flux;
     f.scheme = m->tune.scheme;
+    f.occ4 = m->tune.waves_per_simd == 4;
     f.rare = m->rare;
     f.rec_state = nullptr;
     f.rec_wsum = nullptr;
@@ -288,34 +293,41 @@ template <class R> int prepare_t(hydro_mesh *m) {
     // one primary and one mirror, with exactly opposite normals, is one pair
     // task (listed from its primary slot); every other non-padding slot is a
     // one-slot task.  Within a tile: pairs, then interior slots, then
-    // boundary slots, so most waves run one code path.
+    // boundary slots, so most waves run one code path.  Every tile gets the
+    // same number of entries (the largest task count, rounded up to a
+    // wavefront), so a tile's entries start at t * stride.
     {
         const int B = m->tune.block;
         const int ntiles = (C + B - 1) / B;
-        std::vector<int32_t> toff(ntiles + 1, 0), ta, tb, sgl, bnd;
-        ta.reserve(S4);
-        tb.reserve(S4);
+        std::vector<std::vector<uint32_t>> lists(ntiles);
+        std::vector<uint8_t> pad_n(C, 0);
         int npairs = 0, nsingles = 0;
+        size_t most = 0;
         auto neg_equal = [](R a, R b) {
             const R na = -a;
             return std::memcmp(&na, &b, sizeof(R)) == 0;
         };
+        // slot ids as 4*p + j during planning
+        std::vector<uint32_t> pairs, sgl, bnd;
         for (int t = 0; t < ntiles; t++) {
-            toff[t] = (int32_t)ta.size();
+            pairs.clear();
             sgl.clear();
             bnd.clear();
-            const int pend = std::min(C, (t + 1) * B);
-            for (int p = t * B; p < pend; p++)
+            const int p0 = t * B, pend = std::min(C, (t + 1) * B);
+            for (int p = p0; p < pend; p++)
                 for (int j = 0; j < 4; j++) {
                     const size_t sn = (size_t)j * C + p;
                     const int kind = meta_n[sn] & META_KIND_MASK;
-                    if (kind == K_PAD) continue;
+                    if (kind == K_PAD) {
+                        pad_n[p] |= (uint8_t)(1u << j);
+                        continue;
+                    }
                     if (kind >= K_B10) {
-                        bnd.push_back(4 * p + j);
+                        bnd.push_back(task_word(p - p0, j, 0, 0, false, kind));
                         continue;
                     }
                     const int n = nbr_n[sn];
-                    if (n >= t * B && n < pend && twin_n[sn] != 0xff) {
+                    if (n >= p0 && n < pend && twin_n[sn] != 0xff) {
                         const int k = twin_n[sn];
                         const size_t sk = (size_t)k * C + n;
                         const int kn = meta_n[sk] & META_KIND_MASK;
@@ -325,23 +337,50 @@ template <class R> int prepare_t(hydro_mesh *m) {
                                               neg_equal(cos_n[sn], cos_n[sk]) && neg_equal(sin_n[sn], sin_n[sk]);
                         if (pairable) {
                             if (kind == K_PRIMARY) {
-                                ta.push_back(4 * p + j);
-                                tb.push_back(4 * n + k);
+                                pairs.push_back(task_word(p - p0, j, n - p0, k, true, 0));
                                 npairs++;
                             }
                             continue;
                         }
                     }
-                    sgl.push_back(4 * p + j);
+                    sgl.push_back(task_word(p - p0, j, 0, 0, false, kind));
                 }
-            for (int32_t w : sgl) { ta.push_back(w); tb.push_back(-1); }
-            for (int32_t w : bnd) { ta.push_back(w); tb.push_back(-1); }
+            std::vector<uint32_t> &L = lists[t];
+            L.insert(L.end(), pairs.begin(), pairs.end());
+            L.insert(L.end(), sgl.begin(), sgl.end());
+            L.insert(L.end(), bnd.begin(), bnd.end());
             nsingles += (int)(sgl.size() + bnd.size());
+            most = std::max(most, L.size());
         }
-        toff[ntiles] = (int32_t)ta.size();
+        const size_t stride = std::max<size_t>(64, (most + 63) / 64 * 64);
+        const size_t ntask = stride * (size_t)ntiles;
+        std::vector<uint32_t> tw(ntask, TASK_NONE);
+        std::vector<int32_t> tnb(ntask, -1);
+        std::vector<R> tcs(2 * ntask, (R)0), tsd(2 * ntask, (R)0);
+        for (int t = 0; t < ntiles; t++) {
+            const int p0 = t * B;
+            for (size_t q = 0; q < lists[t].size(); q++) {
+                const uint32_t w = lists[t][q];
+                const size_t k = (size_t)t * stride + q;
+                const int pa = p0 + (int)(w & 0xff), ja = (int)((w >> 8) & 3);
+                const size_t ea = (size_t)ja * C + pa;
+                tw[k] = w;
+                tnb[k] = nbr_n[ea];
+                tcs[2 * k] = cos_n[ea];
+                tcs[2 * k + 1] = sin_n[ea];
+                tsd[2 * k] = side_n[ea];
+                if (w & TASK_PAIR) {
+                    const int pb = p0 + (int)((w >> 10) & 0xff), jb = (int)((w >> 18) & 3);
+                    tsd[2 * k + 1] = side_n[(size_t)jb * C + pb];
+                }
+            }
+        }
         m->n_pairs = npairs;
         m->n_singles = nsingles;
-        if ((rc = h2d(m->task_off, toff)) || (rc = h2d(m->task_a, ta)) || (rc = h2d(m->task_b, tb))) return rc;
+        m->task_stride = (int)stride;
+        if ((rc = h2d(m->task_w, tw)) || (rc = h2d(m->task_nb, tnb)) || (rc = h2d(m->task_cs, tcs)) ||
+            (rc = h2d(m->task_sd, tsd)) || (rc = h2d(m->cell_pad, pad_n)))
+            return rc;
     }
     if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
         (rc = h2d(m->ZB1, zb1_n)) || (rc = h2d(m->nbr, nbr_n)) || (rc = h2d(m->meta, meta_n)) ||
@@ -357,7 +396,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
     for (int b = 0; b < 2; b++)
         if ((rc = m->st[b].alloc((size_t)C * sizeof(State<R>)))) return rc;
     if ((rc = m->W.alloc((size_t)C * sizeof(R)))) return rc;
-    if (!m->tick.p && (rc = m->tick.alloc(sizeof(int64_t)))) return rc;
+    if (!m->tick.p && (rc = m->tick.alloc(sizeof(StepClock)))) return rc;
     if (!m->barrier.p && (rc = m->barrier.alloc(sizeof(unsigned)))) return rc;
     if (!m->perr.p && (rc = m->perr.alloc(sizeof(int)))) return rc;
     HIP_TRY(hipMemset(m->perr.p, 0, sizeof(int)));
@@ -443,11 +482,11 @@ template <class R, int VAR> int build_graph(hydro_mesh *m, int parity, int chunk
     int cur = parity;
     for (int k = 0; k < chunk && err == hipSuccess; k++) {
         const FusedView<R> f = fused_view<R>(m, cur);
-        err = launch_fused_step_tick<R, VAR>(f, m->tick.as<int64_t>(), k, m->spd, m->tune.block,
+        err = launch_fused_step_tick<R, VAR>(f, m->tick.as<StepClock>(), k, m->spd, m->tune.block,
                                              m->tune.xcd_remap, cs);
         cur ^= 1;
     }
-    if (err == hipSuccess) err = launch_tick_advance(m->tick.as<int64_t>(), chunk, cs);
+    if (err == hipSuccess) err = launch_clock_advance(m->tick.as<StepClock>(), chunk, m->spd, cs);
     hipError_t e2 = hipStreamEndCapture(cs, &ge.g);
     if (err != hipSuccess) return fail(HYDRO_ERR_HIP, "graph capture launch failed: %s", hipGetErrorString(err));
     if (e2 != hipSuccess) return fail(HYDRO_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -471,8 +510,7 @@ template <class R, int VAR> int steps_graph(hydro_mesh *m, int n, hipStream_t s)
             if ((rc = build_graph<R, VAR>(m, m->cur, chunk, *ge))) return rc;
             it = m->graphs.emplace(key, std::move(ge)).first;
         }
-        hipLaunchKernelGGL(k_set_tick, dim3(1), dim3(64), 0, s, m->tick.as<int64_t>(), global_step(m));
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(launch_clock_set(m->tick.as<StepClock>(), m->day, m->kt, s));
         while (n - done >= chunk) {
             HIP_TRY(hipGraphLaunch(it->second->x, s));
             for (int k = 0; k < chunk; k++) advance(m);
@@ -628,8 +666,8 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     if (t.block < 64 || t.block > kFusedMaxBlock || (t.block % 64))
         return fail(HYDRO_ERR_INVALID, "block %d (64..%d, multiple of 64)", t.block, kFusedMaxBlock);
     if (t.reorder < 0 || t.reorder > 3) return fail(HYDRO_ERR_INVALID, "reorder %d", t.reorder);
-    if (t.waves_per_simd != 0)
-        return fail(HYDRO_ERR_INVALID, "waves_per_simd %d: only 0 (the compiler's choice)", t.waves_per_simd);
+    if (t.waves_per_simd != 0 && !(t.waves_per_simd == 4 && t.scheme == HYDRO_SCHEME_TILES))
+        return fail(HYDRO_ERR_INVALID, "waves_per_simd %d: 0, or 4 with the TILES scheme", t.waves_per_simd);
     if (t.scheme != HYDRO_SCHEME_TILES && t.scheme != HYDRO_SCHEME_CELL)
         return fail(HYDRO_ERR_INVALID, "scheme %d", t.scheme);
     // the internal state must be written back before the layout is replaced

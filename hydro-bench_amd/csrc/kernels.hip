@@ -127,7 +127,8 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
 
 // ---------------------------------------------------------------------------
 // TILES scheme: one workgroup per tile of blockDim.x consecutive cells.
-//  1. stage the tile's cell states and beds in LDS;
+//  1. stage the tile's cell states and beds in LDS (and prefetch the tile's
+//     task entries and the update's statics into registers);
 //  2. run the tile's tasks: an edge joining two cells of the tile is solved
 //     once for both slots (pair_flux), any other slot alone (slot_flux);
 //     each slot's three flux-sum terms go to LDS;
@@ -154,12 +155,19 @@ template <class R> __device__ __forceinline__ TileLds<R> tile_lds(int B) {
 // Writes the slot's terms to LDS; the flux itself goes to the caller's
 // FLUX0..3 when materialising, or when `keep` (a slot whose code has no
 // handler reads its previous FLUX0/1/3 next step).
+// Writes the slot's terms to LDS; the flux itself goes to the caller's
+// FLUX0..3 when materialising, or when `keep` (a slot whose code has no
+// handler reads its previous FLUX0/1/3 next step).  SL* are SIDE*COSF and
+// SIDE*SINF unless the caller's SLCOS/SLSIN differ from those products.
 template <class R>
-__device__ __forceinline__ void tile_store(const FusedView<R> &f, const TileLds<R> &L, int B, int l, int j, int e,
-                                           int p, const Flux<R> &fl, R c, R s, bool keep = false) {
-    const R SIDE = f.SIDE[e];
-    const R SLCA = f.SLCOS ? f.SLCOS[e] : SIDE * c;
-    const R SLSA = f.SLSIN ? f.SLSIN[e] : SIDE * s;
+__device__ __forceinline__ void tile_store(const FusedView<R> &f, const TileLds<R> &L, int B, int l, int j, int p,
+                                           const Flux<R> &fl, R SIDE, R c, R s, bool keep = false) {
+    R SLCA = SIDE * c, SLSA = SIDE * s;
+    if (f.SLCOS) {
+        const int e = j * f.C + p;
+        SLCA = f.SLCOS[e];
+        SLSA = f.SLSIN[e];
+    }
     const Contrib<R> t = contrib<R>(fl, SIDE, SLCA, SLSA);
     L.c[j * B + l] = t.h;
     L.c[(4 + j) * B + l] = t.u;
@@ -171,6 +179,86 @@ __device__ __forceinline__ void tile_store(const FusedView<R> &f, const TileLds<
     }
 }
 
+// One task entry as loaded (all of it in one coalesced round trip).
+template <class R> struct TaskIn {
+    uint32_t w;
+    int nb;
+    R c, s, sa, sb;
+};
+
+template <class R> __device__ __forceinline__ TaskIn<R> task_load(const FusedView<R> &f, int k) {
+    TaskIn<R> t;
+    t.w = f.task_w[k];
+    t.nb = f.task_nb[k];
+    t.c = f.task_cs[2 * k];
+    t.s = f.task_cs[2 * k + 1];
+    t.sa = f.task_sd[2 * k];
+    t.sb = f.task_sd[2 * k + 1];
+    return t;
+}
+
+template <class R, int VAR, bool RARE>
+__device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R> &L, const TaskIn<R> &t,
+                                         const State<R> *__restrict__ Sin, int B, int p0, int kt, int jt) {
+    const Scalars<R> S = f.S;
+    const int C = f.C;
+    const int la = t.w & 0xff, ja = (t.w >> 8) & 3;
+    const int pa = p0 + la;
+    const State<R> A = L.st[la];
+    const R zbA = L.zb[la];
+    const R zb1A = f.ZB1 ? f.ZB1[pa] : (R)(zbA + S.HM1);
+    if (t.w & TASK_PAIR) {
+        const int lb = (t.w >> 10) & 0xff, jb = (t.w >> 18) & 3;
+        const int pb = p0 + lb;
+        const State<R> M = L.st[lb];
+        const R zbM = L.zb[lb];
+        const R zb1M = f.ZB1 ? f.ZB1[pb] : (R)(zbM + S.HM1);
+        Flux<R> fA, fM;
+        pair_flux<R>(S, A.H, A.U, A.V, A.Z, zb1A, zbA, M.H, M.U, M.V, M.Z, zb1M, zbM, t.c, t.s, &fA, &fM);
+        tile_store<R>(f, L, B, la, ja, pa, fA, t.sa, t.c, t.s);
+        // the mirror slot's normal is exactly the negated one (a pairing condition)
+        tile_store<R>(f, L, B, lb, jb, pb, fM, t.sb, -t.c, -t.s);
+        return;
+    }
+    int kind = (t.w >> 21) & 0xf;
+    int mt = 0;
+    const int ea = ja * C + pa;
+    if (RARE && kind == K_B7) {   // a breach may have opened: its kind is in meta
+        mt = f.meta[ea];
+        kind = mt & META_KIND_MASK;
+    }
+    const int nb = t.nb;
+    const bool has = nb >= 0;
+    State<R> o = {0, 0, 0, 0};
+    R ZBn = 0;
+    if (has) {
+        const unsigned ln = (unsigned)(nb - p0);
+        if (ln < (unsigned)B) {
+            o = L.st[ln];
+            ZBn = L.zb[ln];
+        } else {
+            o = Sin[nb];
+            ZBn = f.ZBC[nb];
+        }
+    }
+    Flux<R> prev = {0, 0, 0, 0};
+    BcCtx<R> bc = empty_bc<R>();
+    if (kind >= K_B10) {
+        const ExtView<R> &m = *f.ext;
+        const int eo = 4 * f.orig[pa] + ja;
+        bc = make_bc<R>(m, f.orig[pa], eo, kt, jt);
+        if (RARE && kind == K_BUNK) prev = {m.F0[eo], m.F1[eo], m.F2[eo], m.F3[eo]};
+    }
+    bool fired = false;
+    const Flux<R> fl = slot_flux<R, VAR, RARE>(kind, S, bc, A.H, A.U, A.V, A.Z, zb1A, zbA, has, o.H, o.U, o.V,
+                                               o.Z, ZBn, t.c, t.s, prev, &fired);
+    if (RARE && fired) {
+        f.meta[ea] = (mt & META_PRIMARY_BIT) ? (uint8_t)(K_PRIMARY | META_PRIMARY_BIT) : (uint8_t)K_MIRROR;
+        f.ext->KLAS[4 * f.orig[pa] + ja] = 0;
+    }
+    tile_store<R>(f, L, B, la, ja, pa, fl, t.sa, t.c, t.s, RARE && kind == K_BUNK);
+}
+
 template <class R, int VAR, bool RARE>
 __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
                                            State<R> *__restrict__ Sout, int t, int kt, int jt) {
@@ -179,80 +267,47 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
     const bool live = p < C;
     const Scalars<R> S = f.S;
     const TileLds<R> L = tile_lds<R>(B);
+    // Phase 1: the cell's state and statics, and the tile's first two task
+    // entries per thread, all independent loads in flight together.
     State<R> me = {0, 0, 0, 0};
-    R zbp = 0;
+    R zbp = 0, area = 0, fnc = 0;
+    R sd[4] = {0, 0, 0, 0};
+    uint32_t pad = 0;
     if (live) {
         me = Sin[p];
         zbp = f.ZBC[p];
+        area = f.AREA[p];
+        fnc = f.FNC[p];
+        pad = f.cell_pad[p];
+        if (VAR == 1) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) sd[j] = f.SIDE[j * C + p];
+        }
+    }
+    const int k0 = t * f.task_stride, kn = k0 + f.task_stride;
+    TaskIn<R> t0, t1;
+    t0.w = t1.w = TASK_NONE;
+    if (k0 + i < kn) t0 = task_load<R>(f, k0 + i);
+    if (k0 + i + B < kn) t1 = task_load<R>(f, k0 + i + B);
+    if (live) {
         L.st[i] = me;
         L.zb[i] = zbp;
     }
     __syncthreads();
-    const int k1 = f.task_off[t + 1];
-    for (int k = f.task_off[t] + i; k < k1; k += B) {
-        const int wa = f.task_a[k], wb = f.task_b[k];
-        const int pa = wa >> 2, ja = wa & 3, la = pa - p0;
-        const int ea = ja * C + pa;
-        const State<R> A = L.st[la];
-        const R zbA = L.zb[la];
-        const R zb1A = f.ZB1 ? f.ZB1[pa] : (R)(zbA + S.HM1);
-        const R ca = f.COSF[ea], sa = f.SINF[ea];
-        if (wb >= 0) {
-            const int pb = wb >> 2, jb = wb & 3, lb = pb - p0;
-            const int eb = jb * C + pb;
-            const State<R> M = L.st[lb];
-            const R zbM = L.zb[lb];
-            const R zb1M = f.ZB1 ? f.ZB1[pb] : (R)(zbM + S.HM1);
-            Flux<R> fA, fM;
-            pair_flux<R>(S, A.H, A.U, A.V, A.Z, zb1A, zbA, M.H, M.U, M.V, M.Z, zb1M, zbM, ca, sa, &fA, &fM);
-            tile_store<R>(f, L, B, la, ja, ea, pa, fA, ca, sa);
-            tile_store<R>(f, L, B, lb, jb, eb, pb, fM, f.COSF[eb], f.SINF[eb]);
-        } else {
-            const int mt = f.meta[ea];
-            const int kind = mt & META_KIND_MASK;
-            const int nb = f.nbr[ea];
-            const bool has = nb >= 0;
-            State<R> o = {0, 0, 0, 0};
-            R ZBn = 0;
-            if (has) {
-                const unsigned ln = (unsigned)(nb - p0);
-                if (ln < (unsigned)B) {
-                    o = L.st[ln];
-                    ZBn = L.zb[ln];
-                } else {
-                    o = Sin[nb];
-                    ZBn = f.ZBC[nb];
-                }
-            }
-            Flux<R> prev = {0, 0, 0, 0};
-            BcCtx<R> bc = empty_bc<R>();
-            if (kind >= K_B10) {
-                const ExtView<R> &m = *f.ext;
-                const int eo = 4 * f.orig[pa] + ja;
-                bc = make_bc<R>(m, f.orig[pa], eo, kt, jt);
-                if (RARE && kind == K_BUNK) prev = {m.F0[eo], m.F1[eo], m.F2[eo], m.F3[eo]};
-            }
-            bool fired = false;
-            const Flux<R> fl = slot_flux<R, VAR, RARE>(kind, S, bc, A.H, A.U, A.V, A.Z, zb1A, zbA, has, o.H, o.U,
-                                                       o.V, o.Z, ZBn, ca, sa, prev, &fired);
-            if (RARE && fired) {
-                f.meta[ea] = (mt & META_PRIMARY_BIT) ? (uint8_t)(K_PRIMARY | META_PRIMARY_BIT)
-                                                     : (uint8_t)K_MIRROR;
-                f.ext->KLAS[4 * f.orig[pa] + ja] = 0;
-            }
-            tile_store<R>(f, L, B, la, ja, ea, pa, fl, ca, sa, RARE && kind == K_BUNK);
-        }
+    // Phase 2: the tasks; each writes its slots' terms to LDS.
+    if (t0.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, t0, Sin, B, p0, kt, jt);
+    if (t1.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, t1, Sin, B, p0, kt, jt);
+    for (int k = k0 + i + 2 * B; k < kn; k += B) {
+        const TaskIn<R> tk = task_load<R>(f, k);
+        if (tk.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, tk, Sin, B, p0, kt, jt);
     }
     __syncthreads();
     if (!live) return;
+    // Phase 3: the cell's four slots in slot order, then its update.
     R WH = 0.0, WU = 0.0, WV = 0.0;
-    R sd[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const int e = j * C + p;
-        const int kind = f.meta[e] & META_KIND_MASK;
-        sd[j] = f.SIDE[e];
-        if (kind == K_PAD) {
+        if ((pad >> j) & 1) {
             const R ZB1 = f.ZB1 ? f.ZB1[p] : (R)(zbp + S.HM1);
             Flux<R> fl;
             if (f.materialize) {
@@ -278,8 +333,8 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
         f.rec_wsum[C + p] = WU;
         f.rec_wsum[2 * C + p] = WV;
     }
-    const CellOut<R> o = update_math<R, VAR>(S, me.H, me.U, me.V, zbp, f.AREA[p], f.FNC[p], WH, WU, WV, sd[0],
-                                             sd[1], sd[2], sd[3]);
+    const CellOut<R> o = update_math<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV, sd[0], sd[1], sd[2],
+                                             sd[3]);
     Sout[p] = State<R>{o.H, o.U, o.V, o.Z};
     f.W[p] = o.W;
 }
@@ -301,6 +356,14 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, i
     step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, jt);
 }
 
+// TILES scheme limited to 128 VGPRs: four waves per SIMD (LDS allows four
+// 256-thread workgroups per CU), for a 207K-cell mesh one resident round.
+template <class R, int VAR, bool RARE>
+__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_fused_step_o4(FusedView<R> f, int kt, int jt, int remap) {
+    tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, jt);
+}
+
 // (day, kt) of global step g counted from day 0, kt 1 (benchmark.cu:89-90).
 __device__ __forceinline__ void time_of(int64_t g, int spd, int *day, int *kt) {
     const int per = spd - 1;
@@ -308,19 +371,48 @@ __device__ __forceinline__ void time_of(int64_t g, int spd, int *day, int *kt) {
     *kt = 1 + (int)(g % per);
 }
 
-// Graph-replayable variant: the step's time comes from a device tick, so the
-// same captured graph advances the boundary series on every replay (the
+// (day, kt) of the step `k` steps after the clock's (one uniform 32-bit
+// division per wave).
+__device__ __forceinline__ void clock_at(StepClock c, int k, int spd, int *day, int *kt) {
+    const int per = spd - 1;
+    const int t = c.kt - 1 + k;
+    *day = c.day + t / per;
+    *kt = 1 + t % per;
+}
+
+// Graph-replayable variant: the step's time comes from a device clock, so
+// the same captured graph advances the boundary series on every replay (the
 // reference's graph replays one frozen (kt=1, day=0) step, benchmark.cu:235-240).
 template <class R, int VAR, bool RARE, int SCHEME>
-__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_tick(FusedView<R> f, const int64_t *tick,
+__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_tick(FusedView<R> f, const StepClock *clock,
                                                                     int offset, int spd, int remap) {
     int day, kt;
-    time_of(*tick + offset, spd, &day, &kt);
+    clock_at(*clock, offset, spd, &day, &kt);
     step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, day);
+}
+
+template <class R, int VAR, bool RARE>
+__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_fused_step_tick_o4(FusedView<R> f, const StepClock *clock, int offset, int spd, int remap) {
+    int day, kt;
+    clock_at(*clock, offset, spd, &day, &kt);
+    tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, day);
 }
 
 __global__ void k_tick_advance(int64_t *tick, int by) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *tick += by;
+}
+
+__global__ void k_clock_advance(StepClock *clock, int by, int spd) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int day, kt;
+        clock_at(*clock, by, spd, &day, &kt);
+        *clock = StepClock{day, kt};
+    }
+}
+
+__global__ void k_clock_set(StepClock *clock, int day, int kt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *clock = StepClock{day, kt};
 }
 
 // ---------------------------------------------------------------------------
@@ -502,23 +594,38 @@ hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, i
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
     FusedView<R> a = f;
     void *args[] = {&a, &kt, &jt, &remap};
-    return hipLaunchKernel(HYDRO_PICK(k_fused_step, R, VAR, f.rare, f.scheme), dim3(div_up(f.C, block)),
-                           dim3(block), args, step_lds<R>(f.scheme, block), s);
+    const void *k = (f.occ4 && f.scheme == 0)
+                        ? (f.rare ? (const void *)k_fused_step_o4<R, VAR, true> : (const void *)k_fused_step_o4<R, VAR, false>)
+                        : HYDRO_PICK(k_fused_step, R, VAR, f.rare, f.scheme);
+    return hipLaunchKernel(k, dim3(div_up(f.C, block)), dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
 template <class R, int VAR>
-hipError_t launch_fused_step_tick(const FusedView<R> &f, const int64_t *tick, int offset, int spd, int block,
+hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock, int offset, int spd, int block,
                                   int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
     if (block > kFusedMaxBlock) return hipErrorInvalidValue;
     FusedView<R> a = f;
-    void *args[] = {&a, &tick, &offset, &spd, &remap};
-    return hipLaunchKernel(HYDRO_PICK(k_fused_step_tick, R, VAR, f.rare, f.scheme), dim3(div_up(f.C, block)),
-                           dim3(block), args, step_lds<R>(f.scheme, block), s);
+    void *args[] = {&a, &clock, &offset, &spd, &remap};
+    const void *k = (f.occ4 && f.scheme == 0)
+                        ? (f.rare ? (const void *)k_fused_step_tick_o4<R, VAR, true>
+                                  : (const void *)k_fused_step_tick_o4<R, VAR, false>)
+                        : HYDRO_PICK(k_fused_step_tick, R, VAR, f.rare, f.scheme);
+    return hipLaunchKernel(k, dim3(div_up(f.C, block)), dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
 hipError_t launch_tick_advance(int64_t *tick, int by, hipStream_t s) {
     hipLaunchKernelGGL(k_tick_advance, dim3(1), dim3(64), 0, s, tick, by);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock_advance(StepClock *clock, int by, int spd, hipStream_t s) {
+    hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, s, clock, by, spd);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock_set(StepClock *clock, int day, int kt, hipStream_t s) {
+    hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, s, clock, day, kt);
     return hipGetLastError();
 }
 
@@ -618,7 +725,7 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
     template hipError_t launch_calculate_flux<R, VAR>(const ExtView<R> &, int, int, int, hipStream_t);   \
     template hipError_t launch_update_cell<R, VAR>(const ExtView<R> &, int, hipStream_t);               \
     template hipError_t launch_fused_step<R, VAR>(const FusedView<R> &, int, int, int, int, hipStream_t); \
-    template hipError_t launch_fused_step_tick<R, VAR>(const FusedView<R> &, const int64_t *, int, int, \
+    template hipError_t launch_fused_step_tick<R, VAR>(const FusedView<R> &, const StepClock *, int, int, \
                                                        int, int, hipStream_t);                          \
     template hipError_t launch_persistent<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, int, \
                                                   hipStream_t);                                         \

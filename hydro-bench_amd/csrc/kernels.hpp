@@ -30,6 +30,7 @@ template <class R> struct FusedView {
     int C;
     int materialize;           // also write the caller's FLUX0..3 every step
     int scheme;                // hydro_scheme: 0 tiles (edge tasks in LDS), 1 one thread per cell
+    int occ4;                  // TILES scheme limited to 128 VGPRs (4 waves per SIMD)
     int rare;                  // mesh has KLAS 3/5/6/7 or unknown codes: launch the full variant
     Scalars<R> S;
     // the caller's arrays, read only by boundary slots, KLAS 7 rewrites and
@@ -49,15 +50,39 @@ template <class R> struct FusedView {
     const R *COSF, *SINF, *SIDE;    // [4][C]
     const R *SLCOS, *SLSIN;    // [4][C] or null when SIDE*COSF / SIDE*SINF reproduce them
     const int32_t *orig;       // [C]
-    // TILES scheme: tile t (cells [t*block, (t+1)*block)) owns the tasks
-    // [task_off[t], task_off[t+1]).  A task is the slot 4*p + j in task_a
-    // and, for an edge inside the tile, the opposite (mirror) slot in
-    // task_b (-1 otherwise); task_a then holds the primary slot.
-    const int32_t *task_off, *task_a, *task_b;
+    // TILES scheme: tile t (cells [t*block, (t+1)*block)) owns the task
+    // entries [t*task_stride, (t+1)*task_stride), in task order:
+    //   task_w   packed word (TaskWord), TASK_NONE for unused entries
+    //   task_nb  new-order neighbour of a one-slot task (-1 none)
+    //   task_cs  [2k] COSF, SINF of the (primary) slot
+    //   task_sd  [2k] SIDE of the slot and of the mirror slot of a pair
+    // cell_pad[p]: bit j set when slot j of cell p is a triangle's padding slot.
+    int task_stride;
+    const uint32_t *task_w;
+    const int32_t *task_nb;
+    const R *task_cs, *task_sd;
+    const uint8_t *cell_pad;
     // recording for the adjoint (null when not recording): the step's input
     // state, and its flux sums WH, WU, WV as [3][C]
     State<R> *rec_state;
     R *rec_wsum;
+};
+
+// Packed task word of the TILES scheme: bits 0-7 local cell a, 8-9 slot of
+// a, 10-17 local cell b, 18-19 slot of b, 20 pair (both slots of an edge
+// inside the tile: a primary, b mirror), 21-24 slot kind of a one-slot task
+// at preparation (KLAS 7 slots are re-read: they may open).
+constexpr uint32_t TASK_NONE = 0xffffffffu;
+constexpr uint32_t TASK_PAIR = 1u << 20;
+__host__ __device__ inline uint32_t task_word(int la, int ja, int lb, int jb, bool pair, int kind) {
+    return (uint32_t)la | ((uint32_t)ja << 8) | ((uint32_t)lb << 10) | ((uint32_t)jb << 18) |
+           (pair ? TASK_PAIR : 0u) | ((uint32_t)kind << 21);
+}
+
+// Time of the next step on a device clock (graph replays advance it):
+// day and kt as the step loop counts them, kt in 1..spd-1.
+struct StepClock {
+    int day, kt;
 };
 
 constexpr int kMaxBlock = 1024;
@@ -75,9 +100,11 @@ hipError_t launch_update_cell(const ExtView<R> &m, int block, hipStream_t s);
 template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int xcd_remap, hipStream_t s);
 template <class R, int VAR>
-hipError_t launch_fused_step_tick(const FusedView<R> &f, const int64_t *tick, int offset, int spd, int block,
+hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock, int offset, int spd, int block,
                                   int xcd_remap, hipStream_t s);
 hipError_t launch_tick_advance(int64_t *tick, int by, hipStream_t s);
+hipError_t launch_clock_advance(StepClock *clock, int by, int spd, hipStream_t s);
+hipError_t launch_clock_set(StepClock *clock, int day, int kt, hipStream_t s);
 
 struct PersistentArgs {
     int nsteps;
