@@ -106,7 +106,7 @@ struct hydro_mesh {
     bool prepared = false;
     int cur = 0;  // state record buffer holding the current state
     bool ext_valid = true, int_valid = false;
-    DevBuf st[2], W, ZBC, AREA, FNC, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
+    DevBuf st[2], ZBC, AREA, FNC, DTA, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
     DevBuf task_w, task_nb, task_cs, task_sd, cell_pad;   // TILES scheme task entries
     int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
@@ -164,10 +164,10 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.ext = m->ext_dev.as<ExtView<R>>();
     f.Sin = m->st[in].as<State<R>>();
     f.Sout = m->st[in ^ 1].as<State<R>>();
-    f.W = m->W.as<R>();
     f.ZBC = m->ZBC.as<R>();
     f.AREA = m->AREA.as<R>();
     f.FNC = m->FNC.as<R>();
+    f.DTA = m->DTA.as<R>();
     f.ZB1 = m->need_zb1 ? m->ZB1.as<R>() : nullptr;
     f.nbr = m->nbr.as<int32_t>();
     f.meta = m->meta.as<uint8_t>();
@@ -185,6 +185,8 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.cell_pad = m->cell_pad.as<uint8_t>();
     f.materialize = m->tune.materialize_flux;
     f.scheme = m->tune.scheme;
+    f.block = m->tune.block;
+    f.ntiles = (m->CELL + m->tune.block - 1) / m->tune.block;
     f.occ4 = m->tune.waves_per_simd == 4;
     f.rare = m->rare;
     f.rec_state = nullptr;
@@ -248,7 +250,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
     m->need_zb1 = !zb1_ok;
     m->need_slcs = !slcs_ok;
 
-    std::vector<R> zbc_n(C), area_n(C), fnc_n(C), zb1_n(m->need_zb1 ? C : 0);
+    std::vector<R> zbc_n(C), area_n(C), fnc_n(C), zb1_n(m->need_zb1 ? C : 0), dta_n(C);
     std::vector<int32_t> nbr_n(S4), orig_n(C);
     std::vector<uint8_t> meta_n(S4);
     std::vector<R> cos_n(S4), sin_n(S4), side_n(S4), slc_n(m->need_slcs ? S4 : 0), sls_n(m->need_slcs ? S4 : 0);
@@ -258,6 +260,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
         orig_n[p] = o;
         zbc_n[p] = ZBC[o];
         area_n[p] = AREA[o];
+        // DT / AREA exactly as the update forms it (1.0 * DT / (1.0 * AREA) in double)
+        dta_n[p] = (R)((1.0 * (double)(R)d.DT) / (1.0 * (double)AREA[o]));
         fnc_n[p] = FNC[o];
         if (m->need_zb1) zb1_n[p] = ZB1[o];
         for (int j = 0; j < 4; j++) {
@@ -353,7 +357,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
             most = std::max(most, L.size());
         }
         const size_t stride = std::max<size_t>(64, (most + 63) / 64 * 64);
-        const size_t ntask = stride * (size_t)ntiles;
+        // 2*B spare entries: the kernel loads two entries per thread unconditionally
+        const size_t ntask = stride * (size_t)ntiles + 2 * (size_t)B;
         std::vector<uint32_t> tw(ntask, TASK_NONE);
         std::vector<int32_t> tnb(ntask, -1);
         std::vector<R> tcs(2 * ntask, (R)0), tsd(2 * ntask, (R)0);
@@ -383,6 +388,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
             return rc;
     }
     if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
+        (rc = h2d(m->DTA, dta_n)) ||
         (rc = h2d(m->ZB1, zb1_n)) || (rc = h2d(m->nbr, nbr_n)) || (rc = h2d(m->meta, meta_n)) ||
         (rc = h2d(m->COSF, cos_n)) || (rc = h2d(m->SINF, sin_n)) || (rc = h2d(m->SIDE, side_n)) ||
         (rc = h2d(m->SLCOS, slc_n)) || (rc = h2d(m->SLSIN, sls_n)) || (rc = h2d(m->orig, orig_n)) ||
@@ -395,7 +401,6 @@ template <class R> int prepare_t(hydro_mesh *m) {
     }
     for (int b = 0; b < 2; b++)
         if ((rc = m->st[b].alloc((size_t)C * sizeof(State<R>)))) return rc;
-    if ((rc = m->W.alloc((size_t)C * sizeof(R)))) return rc;
     if (!m->tick.p && (rc = m->tick.alloc(sizeof(StepClock)))) return rc;
     if (!m->barrier.p && (rc = m->barrier.alloc(sizeof(unsigned)))) return rc;
     if (!m->perr.p && (rc = m->perr.alloc(sizeof(int)))) return rc;
@@ -424,7 +429,7 @@ template <class R> int ensure_internal(hydro_mesh *m, hipStream_t s) {
 template <class R> int ensure_external(hydro_mesh *m, hipStream_t s) {
     if (m->ext_valid) return HYDRO_OK;
     const ExtView<R> ev = ext_view<R>(m);
-    HIP_TRY(launch_scatter<R>(ev, m->orig.as<int32_t>(), m->st[m->cur].as<State<R>>(), m->W.as<R>(), m->CELL, s));
+    HIP_TRY(launch_scatter<R>(ev, m->orig.as<int32_t>(), m->st[m->cur].as<State<R>>(), m->CELL, s));
     m->ext_valid = true;
     return HYDRO_OK;
 }

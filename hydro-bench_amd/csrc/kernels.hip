@@ -121,8 +121,7 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
     }
     const CellOut<R> o = update_math<R, VAR>(S, me.H, me.U, me.V, ZBp, f.AREA[p], f.FNC[p], WH, WU, WV,
                                              sd[0], sd[1], sd[2], sd[3]);
-    Sout[p] = State<R>{o.H, o.U, o.V, o.Z};
-    f.W[p] = o.W;
+    Sout[p] = State<R>{o.H, o.U, o.V, o.Z};   // W = speed(U, V) is formed on the way out
 }
 
 // ---------------------------------------------------------------------------
@@ -139,7 +138,8 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
 
 template <class R> struct TileLds {
     State<R> *st;   // [B]
-    R *zb;          // [B]
+    R *zb;          // [B] bed ZBC
+    R *zb1;         // [B] ZB1
     R *c;           // [3][4][B]: WH, WU, WV terms of slot j of local cell l
 };
 
@@ -148,7 +148,8 @@ template <class R> __device__ __forceinline__ TileLds<R> tile_lds(int B) {
     TileLds<R> L;
     L.st = reinterpret_cast<State<R> *>(hydro_tile_smem);
     L.zb = reinterpret_cast<R *>(hydro_tile_smem + sizeof(State<R>) * B);
-    L.c = L.zb + B;
+    L.zb1 = L.zb + B;
+    L.c = L.zb1 + B;
     return L;
 }
 
@@ -179,12 +180,26 @@ __device__ __forceinline__ void tile_store(const FusedView<R> &f, const TileLds<
     }
 }
 
-// One task entry as loaded (all of it in one coalesced round trip).
+// One task entry as loaded (all of it in one coalesced round trip), and
+// for a one-slot task the neighbour's state and bed.
 template <class R> struct TaskIn {
     uint32_t w;
     int nb;
     R c, s, sa, sb;
+    State<R> on;
+    R zbn;
 };
+
+// Neighbour of a one-slot task, fetched as soon as the entry is in (cell
+// `own` stands in when there is none, so the load is unconditional).
+template <class R>
+__device__ __forceinline__ void task_neighbour(const FusedView<R> &f, const State<R> *__restrict__ Sin, TaskIn<R> &t,
+                                               int own) {
+    const bool single = t.w != TASK_NONE && !(t.w & TASK_PAIR);
+    const int ni = (single && t.nb >= 0) ? t.nb : own;
+    t.on = Sin[ni];
+    t.zbn = f.ZBC[ni];
+}
 
 template <class R> __device__ __forceinline__ TaskIn<R> task_load(const FusedView<R> &f, int k) {
     TaskIn<R> t;
@@ -206,13 +221,13 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
     const int pa = p0 + la;
     const State<R> A = L.st[la];
     const R zbA = L.zb[la];
-    const R zb1A = f.ZB1 ? f.ZB1[pa] : (R)(zbA + S.HM1);
+    const R zb1A = f.ZB1 ? L.zb1[la] : (R)(zbA + S.HM1);
     if (t.w & TASK_PAIR) {
         const int lb = (t.w >> 10) & 0xff, jb = (t.w >> 18) & 3;
         const int pb = p0 + lb;
         const State<R> M = L.st[lb];
         const R zbM = L.zb[lb];
-        const R zb1M = f.ZB1 ? f.ZB1[pb] : (R)(zbM + S.HM1);
+        const R zb1M = f.ZB1 ? L.zb1[lb] : (R)(zbM + S.HM1);
         Flux<R> fA, fM;
         pair_flux<R>(S, A.H, A.U, A.V, A.Z, zb1A, zbA, M.H, M.U, M.V, M.Z, zb1M, zbM, t.c, t.s, &fA, &fM);
         tile_store<R>(f, L, B, la, ja, pa, fA, t.sa, t.c, t.s);
@@ -227,19 +242,12 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
         mt = f.meta[ea];
         kind = mt & META_KIND_MASK;
     }
-    const int nb = t.nb;
-    const bool has = nb >= 0;
+    const bool has = t.nb >= 0;
     State<R> o = {0, 0, 0, 0};
     R ZBn = 0;
     if (has) {
-        const unsigned ln = (unsigned)(nb - p0);
-        if (ln < (unsigned)B) {
-            o = L.st[ln];
-            ZBn = L.zb[ln];
-        } else {
-            o = Sin[nb];
-            ZBn = f.ZBC[nb];
-        }
+        o = t.on;
+        ZBn = t.zbn;
     }
     Flux<R> prev = {0, 0, 0, 0};
     BcCtx<R> bc = empty_bc<R>();
@@ -262,63 +270,73 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
 template <class R, int VAR, bool RARE>
 __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
                                            State<R> *__restrict__ Sout, int t, int kt, int jt) {
-    const int B = blockDim.x, C = f.C, i = threadIdx.x;
+    const int B = f.block, C = f.C, i = threadIdx.x;
     const int p0 = t * B, p = p0 + i;
     const bool live = p < C;
     const Scalars<R> S = f.S;
     const TileLds<R> L = tile_lds<R>(B);
     // Phase 1: the cell's state and statics, and the tile's first two task
     // entries per thread, all independent loads in flight together.
+    // Every load is issued before the first use of any of them, so they are
+    // all in flight together (the compiler orders waits by first use).
     State<R> me = {0, 0, 0, 0};
-    R zbp = 0, area = 0, fnc = 0;
+    R zbp = 0, z1 = 0, area = 0, fnc = 0;
     R sd[4] = {0, 0, 0, 0};
     uint32_t pad = 0;
-    if (live) {
-        me = Sin[p];
-        zbp = f.ZBC[p];
-        area = f.AREA[p];
-        fnc = f.FNC[p];
-        pad = f.cell_pad[p];
-        if (VAR == 1) {
+    const int pc = live ? p : C - 1;   // a valid cell for the unused lanes of the last tile
+    me = Sin[pc];
+    zbp = f.ZBC[pc];
+    // ZB1 is one unconditional load (ZBC again when ZB1 is implied): a
+    // conditional load here would make its wait a wait for everything
+    z1 = (f.ZB1 ? f.ZB1 : f.ZBC)[pc];
+    area = (VAR == 0) ? f.DTA[pc] : f.AREA[pc];   // variant A: DT / AREA itself
+    fnc = f.FNC[pc];
+    pad = f.cell_pad[pc];
+    if (VAR == 1) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) sd[j] = f.SIDE[j * C + p];
-        }
+        for (int j = 0; j < 4; j++) sd[j] = f.SIDE[j * C + pc];
     }
+    // the task arrays carry 2*block spare entries after the last tile, so
+    // both loads are unconditional; entries past this tile's stride are
+    // dropped by a select (no branch around the loads)
     const int k0 = t * f.task_stride, kn = k0 + f.task_stride;
-    TaskIn<R> t0, t1;
-    t0.w = t1.w = TASK_NONE;
-    if (k0 + i < kn) t0 = task_load<R>(f, k0 + i);
-    if (k0 + i + B < kn) t1 = task_load<R>(f, k0 + i + B);
+    TaskIn<R> t0 = task_load<R>(f, k0 + i), t1 = task_load<R>(f, k0 + i + B);
+    if (k0 + i >= kn) t0.w = TASK_NONE;
+    if (k0 + i + B >= kn) t1.w = TASK_NONE;
+    task_neighbour<R>(f, Sin, t0, pc);
+    task_neighbour<R>(f, Sin, t1, pc);
     if (live) {
         L.st[i] = me;
         L.zb[i] = zbp;
+        if (f.ZB1) L.zb1[i] = z1;   // else ZB1 == ZBC + HM1, computed where used
     }
     __syncthreads();
     // Phase 2: the tasks; each writes its slots' terms to LDS.
     if (t0.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, t0, Sin, B, p0, kt, jt);
     if (t1.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, t1, Sin, B, p0, kt, jt);
     for (int k = k0 + i + 2 * B; k < kn; k += B) {
-        const TaskIn<R> tk = task_load<R>(f, k);
+        TaskIn<R> tk = task_load<R>(f, k);
+        task_neighbour<R>(f, Sin, tk, pc);
         if (tk.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, tk, Sin, B, p0, kt, jt);
     }
     __syncthreads();
     if (!live) return;
     // Phase 3: the cell's four slots in slot order, then its update.
+    const R zb1 = f.ZB1 ? L.zb1[i] : (R)(zbp + S.HM1);
     R WH = 0.0, WU = 0.0, WV = 0.0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         if ((pad >> j) & 1) {
-            const R ZB1 = f.ZB1 ? f.ZB1[p] : (R)(zbp + S.HM1);
             Flux<R> fl;
             if (f.materialize) {
                 bool fired = false;
-                fl = slot_flux<R, VAR, RARE>(K_MIRROR, S, empty_bc<R>(), me.H, me.U, me.V, me.Z, ZB1, zbp, false,
+                fl = slot_flux<R, VAR, RARE>(K_MIRROR, S, empty_bc<R>(), me.H, me.U, me.V, me.Z, zb1, zbp, false,
                                              0, 0, 0, 0, 0, (R)0, (R)0, Flux<R>{0, 0, 0, 0}, &fired);
                 const ExtView<R> &m = *f.ext;
                 const int eo = 4 * f.orig[p] + j;
                 m.F0[eo] = fl.f0; m.F1[eo] = fl.f1; m.F2[eo] = fl.f2; m.F3[eo] = fl.f3;
             } else {
-                fl = padding_flux_signs<R>(S, me.H, me.U, me.V, me.Z, ZB1, zbp);
+                fl = padding_flux_signs<R>(S, me.H, me.U, me.V, me.Z, zb1, zbp);
             }
             accumulate<R>(WH, WU, WV, fl, (R)0, (R)0, (R)0);
         } else {
@@ -333,10 +351,10 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
         f.rec_wsum[C + p] = WU;
         f.rec_wsum[2 * C + p] = WV;
     }
-    const CellOut<R> o = update_math<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV, sd[0], sd[1], sd[2],
-                                             sd[3]);
-    Sout[p] = State<R>{o.H, o.U, o.V, o.Z};
-    f.W[p] = o.W;
+    const CellOut<R> o =
+        (VAR == 0) ? update_with_dta<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV)
+                   : update_math<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV, sd[0], sd[1], sd[2], sd[3]);
+    Sout[p] = State<R>{o.H, o.U, o.V, o.Z};   // W = speed(U, V) is formed on the way out
 }
 
 // One step of tile t under either scheme.
@@ -346,14 +364,14 @@ __device__ __forceinline__ void step_tile(const FusedView<R> &f, const State<R> 
     if (SCHEME == 0) {
         tile_cells<R, VAR, RARE>(f, Sin, Sout, t, kt, jt);
     } else {
-        const int p = t * blockDim.x + threadIdx.x;
+        const int p = t * f.block + threadIdx.x;
         if (p < f.C) fused_cell<R, VAR, RARE>(f, Sin, Sout, p, kt, jt);
     }
 }
 
 template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt, int jt, int remap) {
-    step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, jt);
+    step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
 
 // TILES scheme limited to 128 VGPRs: four waves per SIMD (LDS allows four
@@ -361,7 +379,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, i
 template <class R, int VAR, bool RARE>
 __global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_fused_step_o4(FusedView<R> f, int kt, int jt, int remap) {
-    tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, jt);
+    tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
 
 // (day, kt) of global step g counted from day 0, kt 1 (benchmark.cu:89-90).
@@ -388,7 +406,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_tick(FusedView<R>
                                                                     int offset, int spd, int remap) {
     int day, kt;
     clock_at(*clock, offset, spd, &day, &kt);
-    step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, day);
+    step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, day);
 }
 
 template <class R, int VAR, bool RARE>
@@ -396,7 +414,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_pe
 k_fused_step_tick_o4(FusedView<R> f, const StepClock *clock, int offset, int spd, int remap) {
     int day, kt;
     clock_at(*clock, offset, spd, &day, &kt);
-    tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, gridDim.x, remap), kt, day);
+    tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, day);
 }
 
 __global__ void k_tick_advance(int64_t *tick, int by) {
@@ -480,13 +498,13 @@ __device__ __forceinline__ void batch_tile(const BatchView<R> &b, int t, int ste
 
 template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_batch_step(BatchView<R> b, int step, int remap) {
-    batch_tile<R, VAR, RARE, SCHEME>(b, tile_of(blockIdx.x, gridDim.x, remap), step);
+    batch_tile<R, VAR, RARE, SCHEME>(b, tile_of(blockIdx.x, b.ntiles, remap), step);
 }
 
 template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_batch_step_tick(BatchView<R> b, const int64_t *tick, int offset,
                                                                     int remap) {
-    batch_tile<R, VAR, RARE, SCHEME>(b, tile_of(blockIdx.x, gridDim.x, remap), (int)(*tick) + offset);
+    batch_tile<R, VAR, RARE, SCHEME>(b, tile_of(blockIdx.x, b.ntiles, remap), (int)(*tick) + offset);
 }
 
 template <class R, int VAR, bool RARE, int SCHEME>
@@ -512,7 +530,7 @@ __global__ void k_gather(ExtView<R> m, const int32_t *orig, State<R> *st, int C)
 }
 
 template <class R>
-__global__ void k_scatter(ExtView<R> m, const int32_t *orig, const State<R> *st, const R *W, int C) {
+__global__ void k_scatter(ExtView<R> m, const int32_t *orig, const State<R> *st, int C) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= C) return;
     const int o = orig[p];
@@ -521,7 +539,7 @@ __global__ void k_scatter(ExtView<R> m, const int32_t *orig, const State<R> *st,
     m.U[o] = s.U;
     m.V[o] = s.V;
     m.Z[o] = s.Z;
-    m.W[o] = W[p];
+    m.W[o] = speed<R>(s.U, s.V);
 }
 
 __global__ void k_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uint8_t *meta, int C) {
@@ -591,7 +609,7 @@ template <class R> size_t step_lds(int scheme, int block) {
 template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
-    if (block > kFusedMaxBlock) return hipErrorInvalidValue;
+    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;   // tiles are f.block cells
     FusedView<R> a = f;
     void *args[] = {&a, &kt, &jt, &remap};
     const void *k = (f.occ4 && f.scheme == 0)
@@ -604,7 +622,7 @@ template <class R, int VAR>
 hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock, int offset, int spd, int block,
                                   int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
-    if (block > kFusedMaxBlock) return hipErrorInvalidValue;
+    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;
     FusedView<R> a = f;
     void *args[] = {&a, &clock, &offset, &spd, &remap};
     const void *k = (f.occ4 && f.scheme == 0)
@@ -645,7 +663,7 @@ hipError_t persistent_max_grid(int block, bool rare, int scheme, int *grid) {
 
 template <class R, int VAR>
 hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, int block, int grid, hipStream_t s) {
-    if (block > kFusedMaxBlock) return hipErrorInvalidValue;
+    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;
     FusedView<R> a = f;
     PersistentArgs p = pa;
     void *args[] = {&a, &p};
@@ -661,10 +679,9 @@ hipError_t launch_gather(const ExtView<R> &m, const int32_t *orig, State<R> *st,
 }
 
 template <class R>
-hipError_t launch_scatter(const ExtView<R> &m, const int32_t *orig, const State<R> *st, const R *W, int C,
-                          hipStream_t s) {
+hipError_t launch_scatter(const ExtView<R> &m, const int32_t *orig, const State<R> *st, int C, hipStream_t s) {
     if (C == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_scatter<R>), dim3(div_up(C, 256)), dim3(256), 0, s, m, orig, st, W, C);
+    hipLaunchKernelGGL((k_scatter<R>), dim3(div_up(C, 256)), dim3(256), 0, s, m, orig, st, C);
     return hipGetLastError();
 }
 
@@ -746,9 +763,9 @@ template hipError_t launch_gather<double>(const ExtView<double> &, const int32_t
                                           hipStream_t);
 template hipError_t launch_gather<float>(const ExtView<float> &, const int32_t *, State<float> *, int,
                                          hipStream_t);
-template hipError_t launch_scatter<double>(const ExtView<double> &, const int32_t *, const State<double> *,
-                                           const double *, int, hipStream_t);
-template hipError_t launch_scatter<float>(const ExtView<float> &, const int32_t *, const State<float> *,
-                                          const float *, int, hipStream_t);
+template hipError_t launch_scatter<double>(const ExtView<double> &, const int32_t *, const State<double> *, int,
+                                           hipStream_t);
+template hipError_t launch_scatter<float>(const ExtView<float> &, const int32_t *, const State<float> *, int,
+                                          hipStream_t);
 
 }  // namespace hydro

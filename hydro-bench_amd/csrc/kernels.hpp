@@ -30,6 +30,8 @@ template <class R> struct FusedView {
     int C;
     int materialize;           // also write the caller's FLUX0..3 every step
     int scheme;                // hydro_scheme: 0 tiles (edge tasks in LDS), 1 one thread per cell
+    int block;                 // workgroup size = tile size (== blockDim.x of every launch)
+    int ntiles;                // tiles = workgroups of a one-step launch (== gridDim.x)
     int occ4;                  // TILES scheme limited to 128 VGPRs (4 waves per SIMD)
     int rare;                  // mesh has KLAS 3/5/6/7 or unknown codes: launch the full variant
     Scalars<R> S;
@@ -42,8 +44,8 @@ template <class R> struct FusedView {
     // one 32-byte (fp64) / 16-byte (fp32) load instead of four.
     const State<R> *Sin;
     State<R> *Sout;
-    R *W;
     const R *ZBC, *AREA, *FNC;
+    const R *DTA;              // DT / AREA per cell (variant A), as the update forms it
     const R *ZB1;              // null: ZB1 == ZBC + HM1 bitwise for every cell
     const int32_t *nbr;        // [4][C] new-order neighbour, -1 none
     uint8_t *meta;             // [4][C] kind | primary bit (KLAS 7 slots are rewritten)
@@ -56,6 +58,7 @@ template <class R> struct FusedView {
     //   task_nb  new-order neighbour of a one-slot task (-1 none)
     //   task_cs  [2k] COSF, SINF of the (primary) slot
     //   task_sd  [2k] SIDE of the slot and of the mirror slot of a pair
+    // The arrays hold 2*block TASK_NONE entries past the last tile.
     // cell_pad[p]: bit j set when slot j of cell p is a triangle's padding slot.
     int task_stride;
     const uint32_t *task_w;
@@ -89,8 +92,8 @@ constexpr int kMaxBlock = 1024;
 constexpr int kFusedMaxBlock = 256;   // fp64 fused step needs > 128 VGPRs
 
 // Dynamic LDS of a TILES-scheme workgroup of `block` threads: cell states,
-// beds and the three flux-sum contributions of every slot.
-inline size_t tile_lds_bytes(int block, int rsize) { return (size_t)block * 17 * (size_t)rsize; }
+// beds, ZB1 and the three flux-sum contributions of every slot.
+inline size_t tile_lds_bytes(int block, int rsize) { return (size_t)block * 18 * (size_t)rsize; }
 
 // Launch helpers implemented in kernels.hip (explicit instantiations there).
 template <class R, int VAR>
@@ -124,8 +127,7 @@ template <class R>
 hipError_t launch_gather(const ExtView<R> &m, const int32_t *orig, State<R> *st, int C, hipStream_t s);
 // internal records and W -> external H U V Z W
 template <class R>
-hipError_t launch_scatter(const ExtView<R> &m, const int32_t *orig, const State<R> *st, const R *W,
-                          int C, hipStream_t s);
+hipError_t launch_scatter(const ExtView<R> &m, const int32_t *orig, const State<R> *st, int C, hipStream_t s);
 hipError_t launch_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uint8_t *meta, int C,
                                hipStream_t s);
 // ---- batches: many meshes advanced by one launch per step --------------------

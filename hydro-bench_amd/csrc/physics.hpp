@@ -578,12 +578,14 @@ __host__ __device__ __forceinline__ Flux<R> padding_flux_signs(const Scalars<R> 
 // per-cell CFL Fv0:751-768 and 5 m/s cap Fv0:791-793).
 template <class R> struct CellOut { R H, U, V, Z, W; };
 
+// W of UpdateCellKernel (F:783): a function of the new U and V only, so the
+// fused paths recompute it when the state leaves the internal layout.
+template <class R> __host__ __device__ __forceinline__ R speed(R U, R V) { return sqrt_r<R>(U * U + V * V); }
+
+// DT / AREA of the cell: the fixed step (variant A) or the per-cell CFL
+// step of variant B (Fv0:751-768).
 template <class R, int VAR>
-__host__ __device__ __forceinline__ CellOut<R> update_math(const Scalars<R> &S, R H1, R U1, R V1,
-                                                           R ZBp, R AREA, R FNC, R WH, R WU, R WV,
-                                                           R s0, R s1, R s2, R s3) {
-    using K = Consts<R>;
-    R DTA;
+__host__ __device__ __forceinline__ R cell_dta(const Scalars<R> &S, R H1, R U1, R AREA, R s0, R s1, R s2, R s3) {
     if (VAR == 1) {
         R SIDEX;
         if (s3 > (R)0.0) {
@@ -596,10 +598,16 @@ __host__ __device__ __forceinline__ CellOut<R> update_math(const Scalars<R> &S, 
         R DT2 = SIDEX / (U1 + sqrt_r<R>((R)9.81 * HSIDE));
         DT2 = fmin((R)S.DT, DT2);
         DT2 = fmax(DT2, (R)S.DT / (R)10.0);
-        DTA = DT2 / AREA;
-    } else {
-        DTA = 1.0 * S.DT / (1.0 * AREA);
+        return DT2 / AREA;
     }
+    return 1.0 * S.DT / (1.0 * AREA);
+}
+
+// The update given DT / AREA (F:744-783).
+template <class R, int VAR>
+__host__ __device__ __forceinline__ CellOut<R> update_with_dta(const Scalars<R> &S, R H1, R U1, R V1, R ZBp, R DTA,
+                                                               R FNC, R WH, R WU, R WV) {
+    using K = Consts<R>;
     const R WDTA = 1.00 * DTA;
     CellOut<R> o;
     o.H = fmax(H1 - WDTA * WH + K::QLUA, S.HM1);
@@ -620,8 +628,16 @@ __host__ __device__ __forceinline__ CellOut<R> update_math(const Scalars<R> &S, 
         o.U = copysign(fmin(fabs(U2), cap), U2);
         o.V = copysign(fmin(fabs(V2), cap), V2);
     }
-    o.W = sqrt_r<R>(o.U * o.U + o.V * o.V);
+    o.W = speed<R>(o.U, o.V);
     return o;
+}
+
+template <class R, int VAR>
+__host__ __device__ __forceinline__ CellOut<R> update_math(const Scalars<R> &S, R H1, R U1, R V1,
+                                                           R ZBp, R AREA, R FNC, R WH, R WU, R WV,
+                                                           R s0, R s1, R s2, R s3) {
+    return update_with_dta<R, VAR>(S, H1, U1, V1, ZBp, cell_dta<R, VAR>(S, H1, U1, AREA, s0, s1, s2, s3), FNC,
+                                   WH, WU, WV);
 }
 
 // One slot's terms of the cell sums WH, WU, WV (F:732-742).
