@@ -41,7 +41,22 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 512 MiB device-to-device 5 times (a kernel of known traffic, to scale the "
+                         "FETCH_SIZE / WRITE_SIZE counters by)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    n = 64 * 1024 * 1024   # 64 Mi float64 = 512 MiB per tensor
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8, "copies": 5}))
 
 
 def time_gpu(host, mode, steps, warmup, block):
@@ -92,6 +107,9 @@ def cpu_baseline(host, budget_s):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the timestep is not sharded "
               f"in this build", file=sys.stderr)
@@ -116,7 +134,7 @@ def main():
     kernel = {"persistent": "k_persistent<double,0,false,0>", "graph": "k_fused_step_tick<double,0,false,0>",
               "fused": "k_fused_step<double,0,false,0>", "two_kernel": "k_calculate_flux+k_update_cell"}[a.mode]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{a.mode}_{WORKLOAD}_f64.json")
+    tpath = os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{WORKLOAD}_f64_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
