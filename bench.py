@@ -12,6 +12,13 @@ Prints ONE JSON line:
               on a bounded sample of the same workload, 1 host thread
 
 Usage: python bench.py [--gpus 1] [--steps 900] [--warmup 20] [--mode fused]
+
+Two further measurements print their own JSON line instead:
+  --ensemble  128 uneven fp32 meshes (2K-207K cells, 500 steps) advanced as
+              one batch, against the same meshes run one after another with
+              plain launches; per-member accuracy against fp64 single runs
+  --adjoint   forward + adjoint of a 500-step gauge misfit (24,020 cells,
+              checkpoints every 25 steps) against the plain forward run
 """
 import argparse
 import json
@@ -41,6 +48,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--ensemble", action="store_true", help="batched-ensemble speedup instead of the headline line")
+    ap.add_argument("--adjoint", action="store_true", help="adjoint cost ratio instead of the headline line")
+    ap.add_argument("--adjoint-workload", default="floodplain_24020")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 512 MiB device-to-device 5 times (a kernel of known traffic, to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters by)")
@@ -105,6 +115,168 @@ def cpu_baseline(host, budget_s):
     return host.CELL * n / dt, n, dt
 
 
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _timed(fn):
+    """Device time (ms) of the work fn issues on the current stream."""
+    import torch
+
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def ensemble_bench(a):
+    import numpy as np
+
+    import hydro_amd
+    import hydro_amd.mesh as M
+    from hydro_amd.device import DeviceMesh
+    from hydro_amd.ensemble import Batch
+
+    steps = 500
+    t0 = time.perf_counter()
+    # both precisions are built before the GPU is touched (the builder forks)
+    h32 = M.make_ensemble(dtype=np.float32, workers=16)
+    h64 = M.make_ensemble(dtype=np.float64, workers=16)
+    cells = [h.CELL for h in h32]
+    _log(f"ensemble: {len(h32)} meshes, {sum(cells)} cells, built in {time.perf_counter() - t0:.1f} s")
+    tune = hydro_amd.default_tune(block=a.block)
+    dms = [DeviceMesh(h, device=0, tune=tune) for h in h32]
+    inits = [h.state() for h in h32]
+
+    def reset(mode):
+        for dm, st in zip(dms, inits):
+            dm.load_state(st)
+            dm.set_time(0, 1)
+            if mode != "two_kernel":
+                dm.step(0, mode)      # gather into the reordered layout outside the timed region
+
+    def sequential(mode):
+        for dm in dms:
+            dm.step(steps, mode)
+
+    times = {}
+    results = {}
+    for mode in ("two_kernel", "fused"):
+        for rep in range(2):          # the first run warms up
+            reset(mode)
+            times[f"sequential_{mode}"] = _timed(lambda: sequential(mode))
+        for dm in dms:
+            dm.sync()
+        results[mode] = [dm.fetch("H", "U", "V", "Z") for dm in dms]
+        _log(f"sequential {mode}: {times[f'sequential_{mode}']:.1f} ms")
+    b = Batch(dms, tune=tune)
+    same = {}
+    for mode in ("fused", "graph", "persistent"):
+        for rep in range(2):
+            reset("fused")
+            b.step(0, mode)
+            times[f"batched_{mode}"] = _timed(lambda: b.step(steps, mode))
+        b.sync()
+        got = [dm.fetch("H", "U", "V", "Z") for dm in dms]
+        same[mode] = sum(all(np.array_equal(g[k], r[k]) for k in "HUVZ") for g, r in zip(got, results["two_kernel"]))
+        _log(f"batched {mode}: {times[f'batched_{mode}']:.1f} ms, {same[mode]} members bit-equal to sequential")
+    b.close()
+    # accuracy of every member against its own fp64 single-mesh run
+    worst_abs, worst_rel = 0.0, 0.0
+    for i, h in enumerate(h64):
+        ref = DeviceMesh(h, device=0)
+        ref.step(steps, "fused")
+        ref.sync()
+        H64 = ref.fetch("H")["H"]
+        ref.close()
+        H32 = results["two_kernel"][i]["H"].astype(np.float64)
+        d = np.abs(H32 - H64)
+        worst_abs = max(worst_abs, float(d.max()))
+        worst_rel = max(worst_rel, float(d.max() / max(np.abs(H64).max(), 1e-30)))
+    for dm in dms:
+        dm.close()
+    best = min(("fused", "graph", "persistent"), key=lambda m: times[f"batched_{m}"])
+    total = sum(cells) * steps
+    out = {
+        "metric": "batched speedup",
+        "value": times["sequential_two_kernel"] / times[f"batched_{best}"],
+        "unit": "x (sequential plain two-kernel launches / batched)",
+        "n_gpus": 1, "steps": steps, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (128 seeded floodplain meshes, seed 3003, cells log-uniform in [2000, 207234])",
+        "config": {"workload": "ensemble_128", "members": len(cells), "cells": sum(cells), "min_cells": min(cells),
+                   "max_cells": max(cells), "block": a.block, "batched_mode": best},
+        "speedup_vs_sequential_fused": times["sequential_fused"] / times[f"batched_{best}"],
+        "ms": times,
+        "cell_steps_per_s": {k: total / (v / 1e3) for k, v in times.items()},
+        "bit_equal_members": same,
+        "fp32_vs_fp64_max_abs_dH_m": worst_abs,
+        "fp32_vs_fp64_max_rel_dH": worst_rel,
+    }
+    print(json.dumps(out))
+
+
+def adjoint_bench(a):
+    import math
+
+    import numpy as np
+    import torch
+
+    import hydro_amd
+    from hydro_amd.adjoint import MisfitProblem, run_backward, run_forward
+    from hydro_amd.device import DeviceMesh
+
+    host = hydro_amd.make_config(a.adjoint_workload, dtype=np.float64)
+    steps = min(500, host.max_steps)
+    seg = 25 if steps == 500 else int(math.ceil(math.sqrt(steps)))
+    rng = np.random.default_rng(4004)
+    dm = DeviceMesh(host, device=0)
+    dev = dm.device
+    t = lambda x, dt=torch.float64: torch.tensor(x, dtype=dt, device=dev)
+    level = host.Z.copy()
+    gauges = rng.choice(np.nonzero(host.H > 0.5)[0], size=32, replace=False).astype(np.int32)
+    prob = MisfitProblem(dm=dm, level0=t(level), U0=t(host.U), V0=t(host.V), gauges=t(gauges, torch.int32),
+                         steps=steps, segment=seg)
+    n = t(np.sqrt(host.FNC / 9.81))
+    zbc = t(host.ZBC.copy())
+    prob.z_obs = torch.zeros((steps, 32), dtype=torch.float64, device=dev)
+    zg, _ = run_forward(prob, n * 1.05, zbc, keep_checkpoints=False)
+    prob.z_obs = zg.clone()
+    init = host.state()
+
+    def forward_plain():
+        dm.step(steps, "fused")
+
+    res = {}
+    for rep in range(3):
+        dm.load_state(init)
+        dm.set_time(0, 1)
+        dm.step(0, "fused")
+        res["forward"] = _timed(forward_plain)
+        box = {}
+        res["misfit_forward"] = _timed(lambda: box.update(fb=run_forward(prob, n, zbc)))
+        zg, buf = box["fb"]
+        res["adjoint_sweep"] = _timed(lambda: box.update(g=run_backward(prob, buf, zg, n, zbc)))
+    ratio = (res["misfit_forward"] + res["adjoint_sweep"]) / res["forward"]
+    g_n, g_z = box["g"]
+    out = {
+        "metric": "adjoint/forward cost ratio",
+        "value": ratio,
+        "unit": "x ((forward with checkpoints + recompute + adjoint) / plain forward)",
+        "n_gpus": 1, "steps": steps, "higher_is_better": False, "vs_baseline": None, "dtype": "f64",
+        "data": f"synthetic (seeded {host.CELL}-cell floodplain, 32 gauges, truth = Manning x 1.05)",
+        "config": {"workload": a.adjoint_workload, "cells": host.CELL, "steps": steps, "checkpoint_every": seg},
+        "ms": res,
+        "ms_per_step": {k: v / steps for k, v in res.items()},
+        "grad_norms": {"n": float(g_n.norm()), "zbc": float(g_z.norm())},
+    }
+    dm.close()
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -114,6 +286,12 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the timestep is not sharded "
               f"in this build", file=sys.stderr)
         sys.exit(2)
+    if a.ensemble:
+        ensemble_bench(a)
+        return
+    if a.adjoint:
+        adjoint_bench(a)
+        return
     import numpy as np
 
     import hydro_amd

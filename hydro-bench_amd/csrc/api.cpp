@@ -968,6 +968,80 @@ int hydro_gauge_seed(hydro_mesh *m, void *lam, const int32_t *cells, int n, cons
     return HYDRO_OK;
 }
 
+int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const int32_t *gauges, int n, double *zg,
+                         hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc || (rc = check_gauges(m, gauges, n))) return rc;
+    if (nsteps < 0 || seg <= 0 || (n > 0 && !zg)) return fail(HYDRO_ERR_INVALID, "bad argument");
+    const int per = m->spd - 1;
+    const int64_t remaining = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
+    if (nsteps > remaining) return fail(HYDRO_ERR_RANGE, "%d steps requested, %lld remain", nsteps, (long long)remaining);
+    if ((rc = ensure_internal<double>(m, s))) return rc;
+    const size_t C = (size_t)m->CELL;
+    for (int k = 0; k < nsteps; k++) {
+        if (ckpt && k % seg == 0)
+            HIP_TRY(hipMemcpyAsync((State<double> *)ckpt + (size_t)(k / seg) * C, m->st[m->cur].p,
+                                   m->st[m->cur].bytes, hipMemcpyDeviceToDevice, s));
+        const FusedView<double> f = fused_view<double>(m, m->cur);
+        HIP_TRY((launch_fused_step<double, 0>(f, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
+        m->cur ^= 1;
+        advance(m);
+        if (n > 0)
+            HIP_TRY(launch_gauge_gather(m->st[m->cur].as<State<double>>(), m->inv.as<int32_t>(), gauges, n,
+                                        zg + (size_t)k * n, s));
+    }
+    m->ext_valid = false;
+    return HYDRO_OK;
+}
+
+int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, const int32_t *gauges, int n,
+                          const double *resid, void *states, void *wsums, void *lam, double *g_fnc, double *g_zbc,
+                          int *lam_index, hipStream_t s) {
+    int rc = adj_ready(m);
+    if (rc || (rc = check_gauges(m, gauges, n))) return rc;
+    if (nsteps < 0 || seg <= 0 || !ckpt || (n > 0 && !resid) || !states || !wsums || !lam || !g_fnc || !g_zbc ||
+        !lam_index)
+        return fail(HYDRO_ERR_INVALID, "bad argument");
+    const int per = m->spd - 1;
+    if (per <= 0) return fail(HYDRO_ERR_RANGE, "the mesh has no steps per day");
+    const int64_t g0 = global_step(m);
+    if (nsteps > (int64_t)m->d.NDAYS * per - g0)
+        return fail(HYDRO_ERR_RANGE, "%d steps requested past the NDAYS cap", nsteps);
+    if (!m->wbar.p && (rc = m->wbar.alloc((size_t)m->CELL * 3 * sizeof(double)))) return rc;
+    if (!m->adj_flag.p) {
+        if ((rc = m->adj_flag.alloc(sizeof(int)))) return rc;
+        HIP_TRY(hipMemset(m->adj_flag.p, 0, sizeof(int)));
+    }
+    const size_t C = (size_t)m->CELL;
+    State<double> *L[2] = {(State<double> *)lam, (State<double> *)lam + C};
+    HIP_TRY(hipMemsetAsync(L[0], 0, C * sizeof(State<double>), s));
+    int cur = 0;
+    const int nseg = (nsteps + seg - 1) / seg;
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int s0 = k * seg;
+        const int len = std::min(seg, nsteps - s0);
+        HIP_TRY(hipMemcpyAsync(m->st[m->cur].p, (const State<double> *)ckpt + (size_t)k * C, m->st[m->cur].bytes,
+                               hipMemcpyDeviceToDevice, s));
+        m->int_valid = true;
+        m->ext_valid = false;
+        m->day = (int)((g0 + s0) / per);
+        m->kt = 1 + (int)((g0 + s0) % per);
+        if ((rc = record_t<double, 0>(m, len, states, wsums, s))) return rc;
+        const FusedView<double> f = fused_view<double>(m, m->cur);
+        for (int i = len - 1; i >= 0; i--) {
+            const int64_t g = g0 + s0 + i;
+            if (n > 0) HIP_TRY(launch_gauge_seed(L[cur], m->inv.as<int32_t>(), gauges, n, resid + (size_t)(s0 + i) * n, s));
+            HIP_TRY(launch_adjoint_step(f, m->twin.as<uint8_t>(), (const State<double> *)states + (size_t)i * C,
+                                        (const double *)wsums + (size_t)i * 3 * C, L[cur], L[cur ^ 1],
+                                        m->wbar.as<double>(), g_fnc, g_zbc, 1 + (int)(g % per), (int)(g / per),
+                                        m->adj_flag.as<int>(), s));
+            cur ^= 1;
+        }
+    }
+    *lam_index = cur;
+    return HYDRO_OK;
+}
+
 int hydro_cells_map(hydro_mesh *m, const double *in, double *out, int to_external, hipStream_t s) {
     int rc = adj_ready(m);
     if (rc) return rc;

@@ -80,7 +80,7 @@ class _Buffers:
         self.ckpt = torch.empty((nseg + 1, C, 4), dtype=torch.float64, device=dev)
         self.states = torch.empty((seg + 1, C, 4), dtype=torch.float64, device=dev)
         self.wsums = torch.empty((seg, 3, C), dtype=torch.float64, device=dev)
-        self.lam = [torch.zeros((C, 4), dtype=torch.float64, device=dev) for _ in range(2)]
+        self.lam = torch.zeros((2, C, 4), dtype=torch.float64, device=dev)
         self.g_fnc = torch.zeros(C, dtype=torch.float64, device=dev)
         self.g_zbc = torch.zeros(C, dtype=torch.float64, device=dev)
         self.scratch = torch.empty((C, 4), dtype=torch.float64, device=dev)
@@ -109,7 +109,9 @@ def _initial_state(prob: MisfitProblem, zbc):
 
 
 def run_forward(prob: MisfitProblem, n, zbc, keep_checkpoints=True):
-    """Forward run; returns gauge Z [steps, G] and the buffers with checkpoints."""
+    """Forward run (hydro_misfit_forward: one fused step and one gauge read per
+    step, a checkpoint every ``seg`` steps); returns gauge Z [steps, G] and the
+    buffers holding the checkpoints."""
     torch = _torch()
     dm = prob.dm
     lib = dm.lib
@@ -123,47 +125,30 @@ def run_forward(prob: MisfitProblem, n, zbc, keep_checkpoints=True):
     buf = _Buffers(dm, nseg, seg) if keep_checkpoints else None
     G = int(prob.gauges.numel())
     zg = torch.empty((prob.steps, G), dtype=torch.float64, device=dm.device)
-    scratch = buf.scratch if buf is not None else torch.empty((dm.host.CELL, 4), dtype=torch.float64, device=dm.device)
-    for s in range(prob.steps):
-        if buf is not None and s % seg == 0:
-            _lib.check(lib.hydro_state_save(dm.h, _vp(buf.ckpt[s // seg]), st))
-        dm.step(1, "fused")
-        _lib.check(lib.hydro_state_save(dm.h, _vp(scratch), st))
-        _lib.check(lib.hydro_gauge_values(dm.h, _vp(scratch), _vp(prob.gauges), G, _vp(zg[s]), st))
+    ckpt = _vp(buf.ckpt) if buf is not None else ctypes.c_void_p()
+    _lib.check(lib.hydro_misfit_forward(dm.h, prob.steps, seg, ckpt, _vp(prob.gauges), G, _vp(zg), st))
     return zg, buf
 
 
 def run_backward(prob: MisfitProblem, buf: _Buffers, zg, n, zbc, gL=1.0):
-    """Adjoint sweep; returns (dL/dn, dL/dzbc) in the caller's cell order."""
+    """Adjoint sweep (hydro_misfit_backward: per segment, restore the checkpoint,
+    re-run the segment recording step inputs and flux sums, then the adjoint
+    steps backwards); returns (dL/dn, dL/dzbc) in the caller's cell order."""
     torch = _torch()
     dm = prob.dm
     lib = dm.lib
     st = dm.stream()
-    spd = dm.host.steps_per_day
     seg = prob.seg
-    nseg = (prob.steps + seg - 1) // seg
     G = int(prob.gauges.numel())
-    resid = 2.0 * gL * (zg - prob.z_obs)            # dL/dZ at the gauges, per step
-    buf.lam[0].zero_()
+    resid = (2.0 * gL * (zg - prob.z_obs)).contiguous()    # dL/dZ at the gauges, per step
     buf.g_fnc.zero_()
     buf.g_zbc.zero_()
-    cur = 0
-    for k in reversed(range(nseg)):
-        s0 = k * seg
-        m = min(seg, prob.steps - s0)
-        _lib.check(lib.hydro_state_load(dm.h, _vp(buf.ckpt[k]), st))
-        kt, jt = _time_of(s0, spd)
-        dm.set_time(jt, kt)
-        _lib.check(lib.hydro_step_record(dm.h, m, _vp(buf.states), _vp(buf.wsums), st))
-        for i in reversed(range(m)):
-            s = s0 + i
-            # loss term of the state after step s
-            _lib.check(lib.hydro_gauge_seed(dm.h, _vp(buf.lam[cur]), _vp(prob.gauges), G, _vp(resid[s]), st))
-            kt, jt = _time_of(s, spd)
-            _lib.check(lib.hydro_adjoint_step(dm.h, kt, jt, _vp(buf.states[i]), _vp(buf.wsums[i]),
-                                              _vp(buf.lam[cur]), _vp(buf.lam[cur ^ 1]), _vp(buf.g_fnc),
-                                              _vp(buf.g_zbc), st))
-            cur ^= 1
+    dm.set_time(0, 1)
+    idx = ctypes.c_int()
+    _lib.check(lib.hydro_misfit_backward(dm.h, prob.steps, seg, _vp(buf.ckpt), _vp(prob.gauges), G, _vp(resid),
+                                         _vp(buf.states), _vp(buf.wsums), _vp(buf.lam), _vp(buf.g_fnc),
+                                         _vp(buf.g_zbc), ctypes.byref(idx), st))
+    cur = idx.value
     bad = ctypes.c_int()
     _lib.check(lib.hydro_adjoint_check(dm.h, ctypes.byref(bad)))
     if bad.value:

@@ -391,7 +391,7 @@ def make_channel_mesh(seed=1001, cells=6675, steps=100, DT=1.0, MDT=60, dtype=np
 
 def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dtype=np.float64, lib=None,
                          Lx=20000.0, Ly=8000.0, q_peak=400.0, n_bumps=12, n_channel=0.03, n_plain=0.05,
-                         slope=2e-4):
+                         slope=2e-4, depth_offset=0.0):
     """Plain forward, F2-style: a 20 km x 8 km floodplain with a meandering
     main channel carved 3 m into the bed; depths 0-4 m with wet/dry fronts;
     Manning n 0.03 in the channel and 0.05 on the floodplain.
@@ -399,6 +399,8 @@ def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dty
     Mostly triangles (a few hundred quads make the cell count exact).
     Upstream discharge (KLAS 10) enters through the channel mouth, the
     downstream side holds a fixed level (KLAS 1), walls elsewhere.
+    ``depth_offset`` raises the initial level and the downstream level by
+    that many metres (a deeper flood on the same bed).
     """
     rng = np.random.default_rng(seed)
     raw = _raw_mesh(Lx, Ly, cells, 0.0015, rng, aspect=Lx / Ly)
@@ -415,7 +417,7 @@ def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dty
     carve = 3.0 * np.clip(1.0 - (dist / hw) ** 2, 0.0, None)
     in_channel = dist < hw
     bed = plain + bumps * (~in_channel) - carve
-    level = plain + 0.8
+    level = plain + 0.8 + depth_offset
     n = np.where(in_channel, n_channel, n_plain)
     spd = int(MDT / DT)
     NDAYS = math.ceil(steps / max(spd - 1, 1)) + 1
@@ -424,7 +426,7 @@ def make_floodplain_mesh(seed=2002, cells=207234, steps=900, DT=0.5, MDT=60, dty
     y_mouth = 0.5 * Ly + 0.19 * Ly * math.sin(phase)
     inflow = (raw["mx"] < eps) & (np.abs(my - y_mouth) < hw)
     outflow = raw["mx"] > Lx - eps
-    z_out = float(10.0 + 0.8)
+    z_out = float(10.0 + 0.8 + depth_offset)
     q = lambda t: q_peak * (0.75 + 0.25 * math.sin(2.0 * math.pi * t / 3600.0))
     KLAS, qs, zs = _klas_series(raw, Lx, inflow, outflow, q, z_out, 0, rng, NDAYS, MDT)
     raw.update(ZBC=bed, Zinit=level, n=n, KLAS=KLAS, q_series=qs, z_series=zs, NDAYS=NDAYS, MDT=MDT, DT=DT,

@@ -262,6 +262,24 @@ int hydro_states_map(hydro_mesh *m, void *state, double *H, double *U, double *V
 /* re-read ZBC, AREA, FNC (and ZB1 when stored) from the caller's arrays after
  * the caller changed them (e.g. a calibration update) */
 int hydro_refresh_statics(hydro_mesh *m, hipStream_t stream);
+/* Whole-run loops of a gauge misfit, issued from the library with no host
+ * synchronisation inside (the per-call helpers above check the gauge list on
+ * the host each time).
+ * hydro_misfit_forward: nsteps fused steps from the current state and time;
+ *   before step k with k % seg == 0 the state goes to
+ *   ckpt + (k / seg) * state_bytes (ckpt may be NULL); after step k the level
+ *   Z of the n gauges (original ids) goes to zg[k * n + i].
+ * hydro_misfit_backward: the adjoint sweep of such a run.  The mesh's time
+ *   must be the forward run's start time.  resid[k * n + i] = dL/dZ of gauge
+ *   i after step k; states and wsums hold seg records each (hydro_adj_sizes);
+ *   lam holds two state buffers, the first of which is cleared here; g_fnc
+ *   and g_zbc are accumulated (internal order).  On return *lam_index (0 or
+ *   1) names the buffer holding the adjoint of the initial state. */
+int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const int32_t *gauges, int n, double *zg,
+                         hipStream_t stream);
+int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, const int32_t *gauges, int n,
+                          const double *resid, void *states, void *wsums, void *lam, double *g_fnc, double *g_zbc,
+                          int *lam_index, hipStream_t stream);
 
 /* ---- host-side mesh preparation ------------------------------------------ */
 
