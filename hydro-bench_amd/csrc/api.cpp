@@ -111,6 +111,7 @@ struct hydro_mesh {
     int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
+    DevBuf gmap, gnext, gcells;   // misfit loops: gauge map [C], chain [n], internal ids [n]
     DevBuf tick, barrier, perr, ext_dev;   // tick: StepClock of the GRAPH mode
     bool need_zb1 = false, need_slcs = false;
     hipStream_t cap_stream = nullptr;
@@ -191,6 +192,9 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.rare = m->rare;
     f.rec_state = nullptr;
     f.rec_wsum = nullptr;
+    f.gauge_cells = nullptr;
+    f.gauge_n = 0;
+    f.gauge_out = nullptr;
     return f;
 }
 
@@ -968,6 +972,19 @@ int hydro_gauge_seed(hydro_mesh *m, void *lam, const int32_t *cells, int n, cons
     return HYDRO_OK;
 }
 
+// Gauge tables of the misfit loops (internal ids, per-cell first gauge and
+// the chain of repeated ids), rebuilt on the stream for every call.
+static int gauge_tables(hydro_mesh *m, const int32_t *gauges, int n, hipStream_t s) {
+    int rc;
+    if (m->gmap.bytes != (size_t)m->CELL * sizeof(int32_t) && (rc = m->gmap.alloc((size_t)m->CELL * sizeof(int32_t))))
+        return rc;
+    const size_t nb = (size_t)std::max(n, 1) * sizeof(int32_t);
+    if (m->gnext.bytes < nb && ((rc = m->gnext.alloc(nb)) || (rc = m->gcells.alloc(nb)))) return rc;
+    HIP_TRY(launch_gauge_map(m->inv.as<int32_t>(), gauges, n, m->CELL, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
+                             m->gcells.as<int32_t>(), s));
+    return HYDRO_OK;
+}
+
 int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const int32_t *gauges, int n, double *zg,
                          hipStream_t s) {
     int rc = adj_ready(m);
@@ -976,20 +993,25 @@ int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const i
     const int per = m->spd - 1;
     const int64_t remaining = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
     if (nsteps > remaining) return fail(HYDRO_ERR_RANGE, "%d steps requested, %lld remain", nsteps, (long long)remaining);
-    if ((rc = ensure_internal<double>(m, s))) return rc;
+    if ((rc = ensure_internal<double>(m, s)) || (rc = gauge_tables(m, gauges, n, s))) return rc;
     const size_t C = (size_t)m->CELL;
     for (int k = 0; k < nsteps; k++) {
         if (ckpt && k % seg == 0)
             HIP_TRY(hipMemcpyAsync((State<double> *)ckpt + (size_t)(k / seg) * C, m->st[m->cur].p,
                                    m->st[m->cur].bytes, hipMemcpyDeviceToDevice, s));
-        const FusedView<double> f = fused_view<double>(m, m->cur);
+        FusedView<double> f = fused_view<double>(m, m->cur);
+        if (n > 0 && k > 0) {   // this step's input is the previous step's output
+            f.gauge_cells = m->gcells.as<int32_t>();
+            f.gauge_n = n;
+            f.gauge_out = zg + (size_t)(k - 1) * n;
+        }
         HIP_TRY((launch_fused_step<double, 0>(f, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
         m->cur ^= 1;
         advance(m);
-        if (n > 0)
-            HIP_TRY(launch_gauge_gather(m->st[m->cur].as<State<double>>(), m->inv.as<int32_t>(), gauges, n,
-                                        zg + (size_t)k * n, s));
     }
+    if (n > 0 && nsteps > 0)
+        HIP_TRY(launch_gauge_gather(m->st[m->cur].as<State<double>>(), m->inv.as<int32_t>(), gauges, n,
+                                    zg + (size_t)(nsteps - 1) * n, s));
     m->ext_valid = false;
     return HYDRO_OK;
 }
@@ -1007,7 +1029,7 @@ int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, 
     const int64_t g0 = global_step(m);
     if (nsteps > (int64_t)m->d.NDAYS * per - g0)
         return fail(HYDRO_ERR_RANGE, "%d steps requested past the NDAYS cap", nsteps);
-    if (!m->wbar.p && (rc = m->wbar.alloc((size_t)m->CELL * 3 * sizeof(double)))) return rc;
+    if ((rc = gauge_tables(m, gauges, n, s))) return rc;
     if (!m->adj_flag.p) {
         if ((rc = m->adj_flag.alloc(sizeof(int)))) return rc;
         HIP_TRY(hipMemset(m->adj_flag.p, 0, sizeof(int)));
@@ -1030,11 +1052,11 @@ int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, 
         const FusedView<double> f = fused_view<double>(m, m->cur);
         for (int i = len - 1; i >= 0; i--) {
             const int64_t g = g0 + s0 + i;
-            if (n > 0) HIP_TRY(launch_gauge_seed(L[cur], m->inv.as<int32_t>(), gauges, n, resid + (size_t)(s0 + i) * n, s));
-            HIP_TRY(launch_adjoint_step(f, m->twin.as<uint8_t>(), (const State<double> *)states + (size_t)i * C,
-                                        (const double *)wsums + (size_t)i * 3 * C, L[cur], L[cur ^ 1],
-                                        m->wbar.as<double>(), g_fnc, g_zbc, 1 + (int)(g % per), (int)(g / per),
-                                        m->adj_flag.as<int>(), s));
+            HIP_TRY(launch_adjoint_step_fused(f, m->twin.as<uint8_t>(), (const State<double> *)states + (size_t)i * C,
+                                              (const double *)wsums + (size_t)i * 3 * C, L[cur], L[cur ^ 1], g_fnc,
+                                              g_zbc, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
+                                              n > 0 ? resid + (size_t)(s0 + i) * n : nullptr, 1 + (int)(g % per),
+                                              (int)(g / per), m->adj_flag.as<int>(), s));
             cur ^= 1;
         }
     }

@@ -369,8 +369,15 @@ __device__ __forceinline__ void step_tile(const FusedView<R> &f, const State<R> 
     }
 }
 
+// Gauge levels of the step's input state (the misfit loops' per-step read).
+template <class R> __device__ __forceinline__ void gauge_read(const FusedView<R> &f) {
+    if (f.gauge_out != nullptr && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < f.gauge_n; i += blockDim.x) f.gauge_out[i] = (double)f.Sin[f.gauge_cells[i]].Z;
+}
+
 template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt, int jt, int remap) {
+    gauge_read(f);
     step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
 
@@ -379,6 +386,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, i
 template <class R, int VAR, bool RARE>
 __global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_fused_step_o4(FusedView<R> f, int kt, int jt, int remap) {
+    gauge_read(f);
     tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
 

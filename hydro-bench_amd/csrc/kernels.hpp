@@ -69,6 +69,11 @@ template <class R> struct FusedView {
     // state, and its flux sums WH, WU, WV as [3][C]
     State<R> *rec_state;
     R *rec_wsum;
+    // gauge read (null when off): workgroup 0 copies Z of the step's INPUT
+    // state at the internal cells gauge_cells[0..gauge_n) to gauge_out
+    const int32_t *gauge_cells;
+    int gauge_n;
+    double *gauge_out;
 };
 
 // Packed task word of the TILES scheme: bits 0-7 local cell a, 8-9 slot of
@@ -168,6 +173,12 @@ hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, 
 hipError_t launch_tlm_step(const FusedView<double> &f, const State<double> *s_in, const State<double> *ds_in,
                            const double *dfnc, const double *dzbc, State<double> *s_out, State<double> *ds_out, int kt,
                            int jt, hipStream_t s);
+hipError_t launch_adjoint_step_fused(const FusedView<double> &f, const uint8_t *twin, const State<double> *s_in,
+                                     const double *wsum, const State<double> *lam_out, State<double> *lam_in,
+                                     double *g_fnc, double *g_zbc, const int32_t *gmap, const int32_t *gnext,
+                                     const double *seed, int kt, int jt, int *unsupported, hipStream_t s);
+hipError_t launch_gauge_map(const int32_t *inv, const int32_t *cells, int n, int C, int32_t *gmap, int32_t *gnext,
+                            int32_t *internal, hipStream_t s);
 hipError_t launch_gauge_gather(const State<double> *st, const int32_t *inv, const int32_t *cells, int n, double *out,
                                hipStream_t s);
 hipError_t launch_gauge_seed(State<double> *lam, const int32_t *inv, const int32_t *cells, int n, const double *seed,

@@ -115,6 +115,82 @@ def test_adjoint_step_is_transpose_of_tangent_linear():
     dm.close()
 
 
+def test_misfit_loops_match_per_step_api():
+    """hydro_misfit_forward / hydro_misfit_backward (gauge reads inside the
+    step kernel, the one-launch adjoint step with gauge seeds) against the
+    per-step calls (hydro_gauge_values, hydro_step_record, hydro_gauge_seed,
+    the two-kernel hydro_adjoint_step): same bits, with a repeated gauge and
+    a last segment shorter than the others."""
+    import ctypes
+
+    import torch
+
+    import hydro_amd._lib as L
+    from hydro_amd.adjoint import _Buffers, _time_of, run_backward, run_forward
+
+    steps, seg = 60, 8
+    host, prob, n, zbc, rng = _problem(steps=steps, segment=seg)
+    dm = prob.dm
+    lib, st = dm.lib, dm.stream()
+    C = host.CELL
+    dev = dm.device
+    vp = lambda x: ctypes.c_void_p(x.data_ptr())
+    g = prob.gauges.clone()
+    g[5] = g[2]                                   # a repeated gauge id
+    prob.gauges = g
+    G = int(g.numel())
+    prob.z_obs = torch.tensor(rng.normal(11.0, 0.5, (steps, G)), dtype=torch.float64, device=dev)
+    zg, buf = run_forward(prob, n, zbc)
+    g_n, g_z = run_backward(prob, buf, zg, n, zbc)
+    torch.cuda.synchronize()
+    fused_fnc, fused_zbc = buf.g_fnc.clone(), buf.g_zbc.clone()
+
+    # per-step reference
+    from hydro_amd.adjoint import _initial_state, _set_params
+
+    _set_params(dm, n, zbc)
+    H0, Z0, _ = _initial_state(prob, zbc)
+    dm.load_state({"H": H0, "U": prob.U0, "V": prob.V0, "Z": Z0})
+    dm.set_time(0, 1)
+    nseg = (steps + seg - 1) // seg
+    ref = _Buffers(dm, nseg, seg)
+    zg_ref = torch.empty_like(zg)
+    tmp = torch.empty((C, 4), dtype=torch.float64, device=dev)
+    for k in range(steps):
+        if k % seg == 0:
+            L.check(lib.hydro_state_save(dm.h, vp(ref.ckpt[k // seg]), st))
+        dm.step(1, "fused")
+        L.check(lib.hydro_state_save(dm.h, vp(tmp), st))
+        L.check(lib.hydro_gauge_values(dm.h, vp(tmp), vp(g), G, vp(zg_ref[k]), st))
+    torch.cuda.synchronize()
+    assert torch.equal(zg, zg_ref)
+    assert torch.equal(buf.ckpt[:nseg], ref.ckpt[:nseg])
+    resid = (2.0 * (zg_ref - prob.z_obs)).contiguous()
+    ref.lam[0].zero_()
+    ref.g_fnc.zero_()
+    ref.g_zbc.zero_()
+    spd = host.steps_per_day
+    cur = 0
+    for k in reversed(range(nseg)):
+        s0 = k * seg
+        m = min(seg, steps - s0)
+        L.check(lib.hydro_state_load(dm.h, vp(ref.ckpt[k]), st))
+        kt, jt = _time_of(s0, spd)
+        dm.set_time(jt, kt)
+        L.check(lib.hydro_step_record(dm.h, m, vp(ref.states), vp(ref.wsums), st))
+        for i in reversed(range(m)):
+            L.check(lib.hydro_gauge_seed(dm.h, vp(ref.lam[cur]), vp(g), G, vp(resid[s0 + i]), st))
+            kt, jt = _time_of(s0 + i, spd)
+            L.check(lib.hydro_adjoint_step(dm.h, kt, jt, vp(ref.states[i]), vp(ref.wsums[i]), vp(ref.lam[cur]),
+                                           vp(ref.lam[cur ^ 1]), vp(ref.g_fnc), vp(ref.g_zbc), st))
+            cur ^= 1
+    torch.cuda.synchronize()
+    assert torch.equal(fused_fnc, ref.g_fnc)
+    assert torch.equal(fused_zbc, ref.g_zbc)
+    assert float(fused_fnc.abs().max()) > 0 and float(fused_zbc.abs().max()) > 0
+    dm.close()
+
+
 def _gradient_check(deep, n_dirs, fd_tol):
     import torch
 
