@@ -139,21 +139,24 @@ __global__ void k_adj_flux(FusedView<double> f, const uint8_t *twin, const State
 }
 
 // Adjoint step in one launch.  A workgroup covers kAdjCells consecutive
-// cells; wave j (threads 64j .. 64j+63) works on slot j of those cells, so
-// the 4 * C slot reversals run in parallel instead of four per thread.
-//   1. wave 0 reverses the update of its cell (flux-sum adjoints to LDS);
-//      every thread reverses the update of its slot's neighbour itself (the
-//      twin slot's seed), which replaces a separate update pass and a
-//      grid-wide dependency;
-//   2. each thread reverses its own slot's flux (this cell's part) and the
-//      neighbour's twin slot flux (this cell's part, as the neighbour), and
-//      leaves both contributions in LDS;
-//   3. wave 0 adds them to its cell's update adjoint in slot order, the
-//      order k_adj_flux uses, so both paths give the same bits.
+// cells with eight threads per cell: threads 32w .. 32w+31 take slot
+// j = w & 3 of those cells, the own part (w < 4, this cell's slot flux) or
+// the twin part (w >= 4, the neighbour's slot facing this cell), so every
+// thread reverses one slot flux and the two kinds of work sit in different
+// wavefronts.
+//   1. the first own group reverses its cells' update (flux-sum adjoints to
+//      LDS); each twin thread reverses the update of its slot's neighbour
+//      itself (the twin slot's seed), which replaces a separate update pass
+//      and a grid-wide dependency;
+//   2. each thread reverses its slot flux for this cell's part and leaves it
+//      in LDS;
+//   3. the first group adds them to its cell's update adjoint in slot order,
+//      own before twin, the order k_adj_flux uses, so both paths give the
+//      same bits.
 // Gauge seeds enter where lam_out is read: gmap[p] is the first gauge index
 // on cell p (-1 none), gnext chains repeated ids in list order, and
 // seed[i] is added to the Z adjoint in that order (as k_gauge_seed adds it).
-constexpr int kAdjCells = 64;
+constexpr int kAdjCells = 32;
 
 __device__ __forceinline__ double seeded_lz(const State<double> &l, int p, const int32_t *gmap, const int32_t *gnext,
                                             const double *seed) {
@@ -174,7 +177,7 @@ __device__ __forceinline__ UpdateBar cell_bar(const FusedView<double> &f, const 
                       lz);
 }
 
-__global__ void __launch_bounds__(4 * kAdjCells) k_adj_step(FusedView<double> f, const uint8_t *twin,
+__global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<double> f, const uint8_t *twin,
                                                             const State<double> *s_in, const double *wsum,
                                                             const State<double> *lam_out, State<double> *lam_in,
                                                             double *g_fnc, double *g_zbc, const int32_t *gmap,
@@ -184,11 +187,13 @@ __global__ void __launch_bounds__(4 * kAdjCells) k_adj_step(FusedView<double> f,
     __shared__ double part[4][2][5][kAdjCells];
     const int C = f.C;
     const int l = threadIdx.x & (kAdjCells - 1);
-    const int j = threadIdx.x / kAdjCells;
+    const int w = threadIdx.x / kAdjCells;
+    const int j = w & 3;
+    const bool is_twin = w >= 4;
     const int p = blockIdx.x * kAdjCells + l;
     const bool valid = p < C;
     UpdateBar up;
-    if (j == 0 && valid) {
+    if (w == 0 && valid) {
         up = cell_bar(f, s_in, wsum, lam_out, gmap, gnext, seed, p);
         wb[0][l] = up.WH;
         wb[1][l] = up.WU;
@@ -198,7 +203,7 @@ __global__ void __launch_bounds__(4 * kAdjCells) k_adj_step(FusedView<double> f,
     const int kind = valid ? (f.meta[e] & META_KIND_MASK) : K_PAD;
     const int q = (kind != K_PAD) ? f.nbr[e] : -1;
     int j2 = 0xff, kind2 = K_PAD;
-    if (q >= 0) {
+    if (is_twin && q >= 0) {
         j2 = twin[e];
         if (j2 <= 3) kind2 = f.meta[j2 * C + q] & META_KIND_MASK;
     }
@@ -206,9 +211,9 @@ __global__ void __launch_bounds__(4 * kAdjCells) k_adj_step(FusedView<double> f,
     UpdateBar uq;
     if (twin_on) uq = cell_bar(f, s_in, wsum, lam_out, gmap, gnext, seed, q);
     __syncthreads();
-    SideBar own, tw;
+    SideBar acc;
     bool bad = false;
-    if (kind != K_PAD) {
+    if (kind != K_PAD && (!is_twin || twin_on)) {
         const State<double> sp = s_in[p];
         const SideIn me{sp.H, sp.U, sp.V, sp.Z, f.ZBC[p]};
         SideIn nb{0, 0, 0, 0, 0};
@@ -216,56 +221,52 @@ __global__ void __launch_bounds__(4 * kAdjCells) k_adj_step(FusedView<double> f,
             const State<double> sq = s_in[q];
             nb = SideIn{sq.H, sq.U, sq.V, sq.Z, f.ZBC[q]};
         }
-        UpdateBar wp;
-        wp.WH = wb[0][l];
-        wp.WU = wb[1][l];
-        wp.WV = wb[2][l];
-        const double c = f.COSF[e], sn = f.SINF[e], SL = f.SIDE[e];
-        const double SLCA = f.SLCOS ? f.SLCOS[e] : SL * c;
-        const double SLSA = f.SLSIN ? f.SLSIN[e] : SL * sn;
-        double Fb[4];
-        flux_bar_of_sums(wp, SL, SLCA, SLSA, Fb);
-        const BcCtx<double> bc = (kind >= K_B10) ? bc_of(*f.ext, f.orig[p], 4 * f.orig[p] + j, kt, jt) : bc_none();
-        SideBar nbb;
-        bad |= !slot_flux_bar(kind, f.S, bc, me, q >= 0, nb, c, sn, Fb, own, nbb);
-        if (twin_on) {
-            const int e2 = j2 * C + q;
-            const double c2 = f.COSF[e2], s2 = f.SINF[e2], SL2 = f.SIDE[e2];
-            const double SLCA2 = f.SLCOS ? f.SLCOS[e2] : SL2 * c2;
-            const double SLSA2 = f.SLSIN ? f.SLSIN[e2] : SL2 * s2;
-            double Fb2[4];
-            flux_bar_of_sums(uq, SL2, SLCA2, SLSA2, Fb2);
-            const BcCtx<double> bc2 =
-                (kind2 >= K_B10) ? bc_of(*f.ext, f.orig[q], 4 * f.orig[q] + j2, kt, jt) : bc_none();
-            SideBar mb2;
-            bad |= !slot_flux_bar(kind2, f.S, bc2, nb, true, me, c2, s2, Fb2, mb2, tw);
+        // the slot whose flux is reversed: this cell's slot j, or the
+        // neighbour's slot j2 with the roles of the two states swapped
+        const int es = is_twin ? j2 * C + q : e;
+        const int ks = is_twin ? kind2 : kind;
+        UpdateBar ws;
+        if (is_twin) {
+            ws = uq;
+        } else {
+            ws.WH = wb[0][l];
+            ws.WU = wb[1][l];
+            ws.WV = wb[2][l];
         }
+        const double c = f.COSF[es], sn = f.SINF[es], SL = f.SIDE[es];
+        const double SLCA = f.SLCOS ? f.SLCOS[es] : SL * c;
+        const double SLSA = f.SLSIN ? f.SLSIN[es] : SL * sn;
+        double Fb[4];
+        flux_bar_of_sums(ws, SL, SLCA, SLSA, Fb);
+        const int oc = is_twin ? q : p, js = is_twin ? j2 : j;
+        const BcCtx<double> bc = (ks >= K_B10) ? bc_of(*f.ext, f.orig[oc], 4 * f.orig[oc] + js, kt, jt) : bc_none();
+        SideBar other;
+        if (is_twin)
+            bad |= !slot_flux_bar(ks, f.S, bc, nb, true, me, c, sn, Fb, other, acc);
+        else
+            bad |= !slot_flux_bar(ks, f.S, bc, me, q >= 0, nb, c, sn, Fb, acc, other);
     }
-    part[j][0][0][l] = own.H;
-    part[j][0][1][l] = own.U;
-    part[j][0][2][l] = own.V;
-    part[j][0][3][l] = own.Z;
-    part[j][0][4][l] = own.ZB;
-    part[j][1][0][l] = tw.H;
-    part[j][1][1][l] = tw.U;
-    part[j][1][2][l] = tw.V;
-    part[j][1][3][l] = tw.Z;
-    part[j][1][4][l] = tw.ZB;
+    const int k = is_twin ? 1 : 0;
+    part[j][k][0][l] = acc.H;
+    part[j][k][1][l] = acc.U;
+    part[j][k][2][l] = acc.V;
+    part[j][k][3][l] = acc.Z;
+    part[j][k][4][l] = acc.ZB;
     if (bad) *unsupported = 1;
     __syncthreads();
-    if (j != 0 || !valid) return;
+    if (w != 0 || !valid) return;
     State<double> lam{up.H1, up.U1, up.V1, 0.0};
     double gz = 0;
     for (int jj = 0; jj < 4; jj++) {
-        for (int k = 0; k < 2; k++) {
+        for (int kk = 0; kk < 2; kk++) {
             // a padding slot or a missing twin contributes exact zeros, which
             // k_adj_flux skips; adding +0.0 leaves every sum's bits unchanged
             // except -0.0 + 0.0, and lam starts from the update's adjoints
-            lam.H += part[jj][k][0][l];
-            lam.U += part[jj][k][1][l];
-            lam.V += part[jj][k][2][l];
-            lam.Z += part[jj][k][3][l];
-            gz += part[jj][k][4][l];
+            lam.H += part[jj][kk][0][l];
+            lam.U += part[jj][kk][1][l];
+            lam.V += part[jj][kk][2][l];
+            lam.Z += part[jj][kk][3][l];
+            gz += part[jj][kk][4][l];
         }
     }
     lam_in[p] = lam;
@@ -422,7 +423,7 @@ hipError_t launch_adjoint_step_fused(const FusedView<double> &f, const uint8_t *
                                      double *g_fnc, double *g_zbc, const int32_t *gmap, const int32_t *gnext,
                                      const double *seed, int kt, int jt, int *unsupported, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_step, dim3(div_up(f.C, kAdjCells)), dim3(4 * kAdjCells), 0, s, f, twin, s_in, wsum,
+    hipLaunchKernelGGL(k_adj_step, dim3(div_up(f.C, kAdjCells)), dim3(8 * kAdjCells), 0, s, f, twin, s_in, wsum,
                        lam_out, lam_in, g_fnc, g_zbc, gmap, gnext, seed, kt, jt, unsupported);
     return hipGetLastError();
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -108,6 +109,10 @@ struct hydro_mesh {
     bool ext_valid = true, int_valid = false;
     DevBuf st[2], ZBC, AREA, FNC, DTA, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
     DevBuf task_w, task_nb, task_cs, task_sd, cell_pad;   // TILES scheme task entries
+    DevBuf tile_nb_off, tile_nb, tile_flag;   // persistent TILES kernel: neighbour tiles (CSR), step flags
+    int max_tile_nb = 0;
+    int tile = 256;                 // cells per tile (resolved hydro_tune.tile_cells)
+    bool persist_pending = false;   // a persistent launch whose wait flag has not been read back
     int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
@@ -187,11 +192,15 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.materialize = m->tune.materialize_flux;
     f.scheme = m->tune.scheme;
     f.block = m->tune.block;
-    f.ntiles = (m->CELL + m->tune.block - 1) / m->tune.block;
+    f.tile = m->tile;
+    f.ntiles = (m->CELL + m->tile - 1) / m->tile;
     f.occ4 = m->tune.waves_per_simd == 4;
     f.rare = m->rare;
     f.rec_state = nullptr;
     f.rec_wsum = nullptr;
+    f.tile_nb_off = m->tile_nb_off.as<int32_t>();
+    f.tile_nb = m->tile_nb.as<int32_t>();
+    f.tile_flag = m->tile_flag.as<int32_t>();
     f.gauge_cells = nullptr;
     f.gauge_n = 0;
     f.gauge_out = nullptr;
@@ -229,7 +238,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
     if (m->tune.reorder == HYDRO_REORDER_RCM) {
         perm = rcm_order(C, NAC.data());
     } else if (m->tune.reorder == HYDRO_REORDER_TILES) {
-        perm = tile_order(C, NAC.data(), m->tune.block, rcm_order(C, NAC.data()));
+        perm = tile_order(C, NAC.data(), m->tile, rcm_order(C, NAC.data()));
     } else if (m->tune.reorder == HYDRO_REORDER_NONE) {
         perm.resize(C);
         for (int i = 0; i < C; i++) perm[i] = i;
@@ -305,7 +314,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
     // same number of entries (the largest task count, rounded up to a
     // wavefront), so a tile's entries start at t * stride.
     {
-        const int B = m->tune.block;
+        const int B = m->tile;
         const int ntiles = (C + B - 1) / B;
         std::vector<std::vector<uint32_t>> lists(ntiles);
         std::vector<uint8_t> pad_n(C, 0);
@@ -361,8 +370,10 @@ template <class R> int prepare_t(hydro_mesh *m) {
             most = std::max(most, L.size());
         }
         const size_t stride = std::max<size_t>(64, (most + 63) / 64 * 64);
-        // 2*B spare entries: the kernel loads two entries per thread unconditionally
-        const size_t ntask = stride * (size_t)ntiles + 2 * (size_t)B;
+        // spare entries: every thread of the last tile's workgroup loads
+        // entries k0 + i and k0 + i + block unconditionally (block threads,
+        // which may be more than the tile's cells)
+        const size_t ntask = stride * (size_t)ntiles + 2 * (size_t)m->tune.block;
         std::vector<uint32_t> tw(ntask, TASK_NONE);
         std::vector<int32_t> tnb(ntask, -1);
         std::vector<R> tcs(2 * ntask, (R)0), tsd(2 * ntask, (R)0);
@@ -389,6 +400,32 @@ template <class R> int prepare_t(hydro_mesh *m) {
         m->task_stride = (int)stride;
         if ((rc = h2d(m->task_w, tw)) || (rc = h2d(m->task_nb, tnb)) || (rc = h2d(m->task_cs, tcs)) ||
             (rc = h2d(m->task_sd, tsd)) || (rc = h2d(m->cell_pad, pad_n)))
+            return rc;
+    }
+    {
+        // tiles (workgroups of `block` consecutive cells) that share an edge:
+        // the persistent TILES kernel waits on exactly these between steps
+        const int B = m->tile, nt = (C + B - 1) / B;
+        std::vector<std::vector<int32_t>> adj(nt);
+        for (int j = 0; j < 4; j++)
+            for (int p = 0; p < C; p++) {
+                const int q = nbr_n[(size_t)j * C + p];
+                if (q < 0 || q / B == p / B) continue;
+                adj[p / B].push_back(q / B);
+                adj[q / B].push_back(p / B);
+            }
+        std::vector<int32_t> off(nt + 1, 0), lst;
+        for (int t = 0; t < nt; t++) {
+            std::sort(adj[t].begin(), adj[t].end());
+            adj[t].erase(std::unique(adj[t].begin(), adj[t].end()), adj[t].end());
+            lst.insert(lst.end(), adj[t].begin(), adj[t].end());
+            off[t + 1] = (int32_t)lst.size();
+        }
+        m->max_tile_nb = 0;
+        for (int t = 0; t < nt; t++) m->max_tile_nb = std::max(m->max_tile_nb, off[t + 1] - off[t]);
+        if (lst.empty()) lst.push_back(0);
+        if ((rc = h2d(m->tile_nb_off, off)) || (rc = h2d(m->tile_nb, lst)) ||
+            (rc = m->tile_flag.alloc((size_t)nt * sizeof(int32_t))))
             return rc;
     }
     if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
@@ -430,7 +467,25 @@ template <class R> int ensure_internal(hydro_mesh *m, hipStream_t s) {
     return HYDRO_OK;
 }
 
+// A persistent launch whose waits exceeded their spin limit left early with
+// a partial state: report it at the next call that reads the state or
+// launches again (one synchronous 4-byte read per persistent launch).
+int check_persistent(hydro_mesh *m, hipStream_t s) {
+    if (!m->persist_pending) return HYDRO_OK;
+    m->persist_pending = false;
+    int err = 0;
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(&err, m->perr.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        HIP_TRY(hipMemset(m->perr.p, 0, sizeof(int)));
+        return fail(HYDRO_ERR_HIP, "a persistent launch stopped: a wait between steps exceeded its spin limit");
+    }
+    return HYDRO_OK;
+}
+
 template <class R> int ensure_external(hydro_mesh *m, hipStream_t s) {
+    int rc = check_persistent(m, s);
+    if (rc) return rc;
     if (m->ext_valid) return HYDRO_OK;
     const ExtView<R> ev = ext_view<R>(m);
     HIP_TRY(launch_scatter<R>(ev, m->orig.as<int32_t>(), m->st[m->cur].as<State<R>>(), m->CELL, s));
@@ -533,13 +588,13 @@ template <class R, int VAR> int steps_graph(hydro_mesh *m, int n, hipStream_t s)
 }
 
 template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream_t s) {
-    int rc = ensure_internal<R>(m, s);
-    if (rc) return rc;
+    int rc = check_persistent(m, s);
+    if (rc || (rc = ensure_internal<R>(m, s))) return rc;
     if (n == 0) return HYDRO_OK;
     const int block = m->tune.block;
     int maxg = 0;
-    HIP_TRY((persistent_max_grid<R, VAR>(block, m->rare, m->tune.scheme, &maxg)));
-    const int ntiles = (m->CELL + block - 1) / block;
+    HIP_TRY((persistent_max_grid<R, VAR>(block, m->rare, m->tune.scheme, m->tune.waves_per_simd == 4, &maxg)));
+    const int ntiles = (m->CELL + m->tile - 1) / m->tile;
     int want = m->tune.persistent_wgs_per_cu;
     int grid = maxg;
     if (want > 0) {
@@ -560,7 +615,9 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
     pa.spin_limit = 1u << 24;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
     HIP_TRY(hipGetLastError());
+    if (f.scheme == HYDRO_SCHEME_TILES) HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles, s));
     HIP_TRY((launch_persistent<R, VAR>(f, pa, block, grid, s)));
+    m->persist_pending = true;
     if (n & 1) m->cur ^= 1;
     for (int i = 0; i < n; i++) advance(m);
     m->ext_valid = false;
@@ -588,6 +645,7 @@ hydro_tune default_tune() {
     t.materialize_flux = 0;
     t.waves_per_simd = 0;
     t.scheme = HYDRO_SCHEME_TILES;
+    t.tile_cells = 0;
     return t;
 }
 
@@ -679,6 +737,22 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
         return fail(HYDRO_ERR_INVALID, "waves_per_simd %d: 0, or 4 with the TILES scheme", t.waves_per_simd);
     if (t.scheme != HYDRO_SCHEME_TILES && t.scheme != HYDRO_SCHEME_CELL)
         return fail(HYDRO_ERR_INVALID, "scheme %d", t.scheme);
+    if (t.tile_cells != 0 && (t.tile_cells < 64 || t.tile_cells > t.block))
+        return fail(HYDRO_ERR_INVALID, "tile_cells %d (0, or 64..block)", t.tile_cells);
+    int tile = t.block;
+    if (t.scheme == HYDRO_SCHEME_TILES && t.tile_cells > 0) {
+        tile = t.tile_cells;
+    } else if (t.scheme == HYDRO_SCHEME_TILES) {
+        // a mesh with fewer full tiles than CUs leaves CUs idle: cut it into
+        // one tile per CU instead (a tile's time is mostly the latency of its
+        // edge solves, which smaller tiles shorten)
+        int dev = 0, cus = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int C = m->CELL;
+        if (cus > 0 && C > 0 && ((int64_t)C + t.block - 1) / t.block < cus)
+            tile = (int)std::max<int64_t>(64, ((int64_t)C + cus - 1) / cus);
+    }
     // the internal state must be written back before the layout is replaced
     int rc = HYDRO_OK;
     if (m->prepared && !m->ext_valid)
@@ -686,6 +760,7 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     if (rc) return rc;
     HIP_TRY(hipDeviceSynchronize());
     m->tune = t;
+    m->tile = tile;
     m->prepared = false;
     return (m->d.dtype == HYDRO_F64) ? prepare_t<double>(m) : prepare_t<float>(m);
 }
@@ -1231,6 +1306,7 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
     b->remap = t.xcd_remap;
     b->chunk = t.graph_chunk;
     b->scheme = t.scheme;
+    if (t.tile_cells == 0) t.tile_cells = t.block;   // members keep whole tiles: the batch balances them
     std::vector<int32_t> tile_mesh, tile0(n);
     for (int i = 0; i < n; i++) {
         hydro_mesh *m = meshes[i];
@@ -1239,13 +1315,14 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
             return fail(HYDRO_ERR_INVALID, "mesh %d: every mesh of a batch needs the same dtype and variant", i);
         for (int j = 0; j < i; j++)
             if (meshes[j] == m) return fail(HYDRO_ERR_INVALID, "mesh %d appears twice", i);
-        if (!m->prepared || m->tune.block != t.block || m->tune.reorder != t.reorder ||
+        if (!m->prepared || m->tune.block != t.block || m->tune.tile_cells != t.tile_cells ||
+            m->tune.reorder != t.reorder ||
             m->tune.materialize_flux != t.materialize_flux || m->tune.scheme != t.scheme) {
             int rc = hydro_prepare(m, &t);
             if (rc) return rc;
         }
         tile0[i] = (int32_t)tile_mesh.size();
-        const int nt = (m->CELL + t.block - 1) / t.block;
+        const int nt = (m->CELL + m->tile - 1) / m->tile;
         for (int k = 0; k < nt; k++) tile_mesh.push_back(i);
         b->meshes.push_back(m);
         b->rare = b->rare || m->rare;

@@ -180,6 +180,55 @@ __device__ __forceinline__ void tile_store(const FusedView<R> &f, const TileLds<
     }
 }
 
+// State records as the persistent TILES kernel moves them between
+// workgroups: buffer loads and stores with the sc1 cache policy, i.e.
+// coherent across the XCDs' L2 caches (what device-scope relaxed atomics
+// compile to, but 16 bytes per instruction), so a step's output is visible
+// to a neighbouring tile on another XCD once its flag is, with no
+// cache-wide write-back or invalidate.  Other launches use plain accesses.
+typedef int hydro_i4 __attribute__((ext_vector_type(4)));
+constexpr int kBufSC1 = 16;                  // buffer cache-policy operand: sc1
+constexpr int kBufWord3 = 0x00020000;        // gfx9 buffer descriptor word 3 (32-bit data format)
+
+template <class R> __device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc(const State<R> *S, int C) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<State<R> *>(S), 0, C * (int)sizeof(State<R>), kBufWord3);
+}
+
+template <bool COH, class R> __device__ __forceinline__ State<R> ld_state(const State<R> *S, int i, int C) {
+    if (!COH) return S[i];
+    const __amdgpu_buffer_rsrc_t r = state_rsrc(S, C);
+    const int off = i * (int)sizeof(State<R>);
+    if constexpr (sizeof(R) == 8) {
+        const hydro_i4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kBufSC1);
+        const hydro_i4 b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, kBufSC1);
+        struct P {
+            hydro_i4 a, b;
+        } pk{a, b};
+        return __builtin_bit_cast(State<R>, pk);
+    } else {
+        return __builtin_bit_cast(State<R>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kBufSC1));
+    }
+}
+
+template <bool COH, class R> __device__ __forceinline__ void st_state(State<R> *S, int i, int C, const State<R> &v) {
+    if (!COH) {
+        S[i] = v;
+        return;
+    }
+    const __amdgpu_buffer_rsrc_t r = state_rsrc(S, C);
+    const int off = i * (int)sizeof(State<R>);
+    if constexpr (sizeof(R) == 8) {
+        struct P {
+            hydro_i4 a, b;
+        };
+        const P pk = __builtin_bit_cast(P, v);
+        __builtin_amdgcn_raw_buffer_store_b128(pk.a, r, off, 0, kBufSC1);
+        __builtin_amdgcn_raw_buffer_store_b128(pk.b, r, off + 16, 0, kBufSC1);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(hydro_i4, v), r, off, 0, kBufSC1);
+    }
+}
+
 // One task entry as loaded (all of it in one coalesced round trip), and
 // for a one-slot task the neighbour's state and bed.
 template <class R> struct TaskIn {
@@ -192,12 +241,12 @@ template <class R> struct TaskIn {
 
 // Neighbour of a one-slot task, fetched as soon as the entry is in (cell
 // `own` stands in when there is none, so the load is unconditional).
-template <class R>
+template <bool COH, class R>
 __device__ __forceinline__ void task_neighbour(const FusedView<R> &f, const State<R> *__restrict__ Sin, TaskIn<R> &t,
                                                int own) {
     const bool single = t.w != TASK_NONE && !(t.w & TASK_PAIR);
     const int ni = (single && t.nb >= 0) ? t.nb : own;
-    t.on = Sin[ni];
+    t.on = ld_state<COH>(Sin, ni, f.C);
     t.zbn = f.ZBC[ni];
 }
 
@@ -267,12 +316,12 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
     tile_store<R>(f, L, B, la, ja, pa, fl, t.sa, t.c, t.s, RARE && kind == K_BUNK);
 }
 
-template <class R, int VAR, bool RARE>
+template <class R, int VAR, bool RARE, bool COH = false>
 __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
                                            State<R> *__restrict__ Sout, int t, int kt, int jt) {
     const int B = f.block, C = f.C, i = threadIdx.x;
-    const int p0 = t * B, p = p0 + i;
-    const bool live = p < C;
+    const int p0 = t * f.tile, p = p0 + i;
+    const bool live = i < f.tile && p < C;
     const Scalars<R> S = f.S;
     const TileLds<R> L = tile_lds<R>(B);
     // Phase 1: the cell's state and statics, and the tile's first two task
@@ -284,7 +333,7 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
     R sd[4] = {0, 0, 0, 0};
     uint32_t pad = 0;
     const int pc = live ? p : C - 1;   // a valid cell for the unused lanes of the last tile
-    me = Sin[pc];
+    me = ld_state<COH>(Sin, pc, C);
     zbp = f.ZBC[pc];
     // ZB1 is one unconditional load (ZBC again when ZB1 is implied): a
     // conditional load here would make its wait a wait for everything
@@ -303,8 +352,8 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
     TaskIn<R> t0 = task_load<R>(f, k0 + i), t1 = task_load<R>(f, k0 + i + B);
     if (k0 + i >= kn) t0.w = TASK_NONE;
     if (k0 + i + B >= kn) t1.w = TASK_NONE;
-    task_neighbour<R>(f, Sin, t0, pc);
-    task_neighbour<R>(f, Sin, t1, pc);
+    task_neighbour<COH>(f, Sin, t0, pc);
+    task_neighbour<COH>(f, Sin, t1, pc);
     if (live) {
         L.st[i] = me;
         L.zb[i] = zbp;
@@ -316,7 +365,7 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
     if (t1.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, t1, Sin, B, p0, kt, jt);
     for (int k = k0 + i + 2 * B; k < kn; k += B) {
         TaskIn<R> tk = task_load<R>(f, k);
-        task_neighbour<R>(f, Sin, tk, pc);
+        task_neighbour<COH>(f, Sin, tk, pc);
         if (tk.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, tk, Sin, B, p0, kt, jt);
     }
     __syncthreads();
@@ -354,7 +403,7 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
     const CellOut<R> o =
         (VAR == 0) ? update_with_dta<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV)
                    : update_math<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV, sd[0], sd[1], sd[2], sd[3]);
-    Sout[p] = State<R>{o.H, o.U, o.V, o.Z};   // W = speed(U, V) is formed on the way out
+    st_state<COH>(Sout, p, C, State<R>{o.H, o.U, o.V, o.Z});   // W = speed(U, V) is formed on the way out
 }
 
 // One step of tile t under either scheme.
@@ -468,6 +517,67 @@ __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned target,
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
+}
+
+// TILES scheme: no grid-wide barrier.  Tile t starts step s once every tile
+// sharing an edge with it has finished step s-1 (its flag reached s): that
+// covers both the halo it reads (their step s-1 output) and the buffer it
+// overwrites (their step s-1 input, which they no longer read).  States move
+// with device-coherent accesses, so publishing a tile is: every wave waits
+// for its stores, then one relaxed flag store.  Neighbouring tiles stay
+// within one step of each other; distant ones may drift further.
+// Every workgroup takes tiles blockIdx.x, blockIdx.x + gridDim.x, ... in
+// step-major order; a wait only ever points at an earlier step, so with all
+// workgroups resident (cooperative launch) every wait ends.  A wait longer
+// than spin_limit sleeps sets *error and the kernel leaves.
+template <class R>
+__device__ __forceinline__ bool wait_tiles(const FusedView<R> &f, int t, int target, const PersistentArgs &pa) {
+    const int a = f.tile_nb_off[t], b = f.tile_nb_off[t + 1];
+    int ok = 1;
+    for (int k = a + (int)threadIdx.x; k < b; k += blockDim.x) {
+        const int u = f.tile_nb[k];
+        unsigned spins = 0;
+        while (__hip_atomic_load(f.tile_flag + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > pa.spin_limit || __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+        }
+    }
+    return __syncthreads_and(ok) != 0;
+}
+
+template <class R, int VAR, bool RARE>
+__device__ __forceinline__ void persistent_tiles(const FusedView<R> &f, const PersistentArgs &pa) {
+    const int ntiles = f.ntiles;
+    const int64_t g0 = (int64_t)pa.day0 * (pa.spd - 1) + (pa.kt0 - 1);
+    for (int s = 0; s < pa.nsteps; s++) {
+        int day, kt;
+        time_of(g0 + s, pa.spd, &day, &kt);
+        const State<R> *a = (s & 1) ? f.Sout : f.Sin;
+        State<R> *b = (s & 1) ? const_cast<State<R> *>(f.Sin) : f.Sout;
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+            if (s > 0 && !wait_tiles(f, t, s, pa)) return;
+            tile_cells<R, VAR, RARE, true>(f, a, b, t, kt, day);
+            __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are complete
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(f.tile_flag + t, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <class R, int VAR, bool RARE>
+__global__ void __launch_bounds__(kFusedMaxBlock) k_persistent_tiles(FusedView<R> f, PersistentArgs pa) {
+    persistent_tiles<R, VAR, RARE>(f, pa);
+}
+
+// the same limited to 128 VGPRs (four workgroups per CU, some spills)
+template <class R, int VAR, bool RARE>
+__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_persistent_tiles_o4(FusedView<R> f, PersistentArgs pa) {
+    persistent_tiles<R, VAR, RARE>(f, pa);
 }
 
 template <class R, int VAR, bool RARE, int SCHEME>
@@ -617,13 +727,13 @@ template <class R> size_t step_lds(int scheme, int block) {
 template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
-    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;   // tiles are f.block cells
+    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;
     FusedView<R> a = f;
     void *args[] = {&a, &kt, &jt, &remap};
     const void *k = (f.occ4 && f.scheme == 0)
                         ? (f.rare ? (const void *)k_fused_step_o4<R, VAR, true> : (const void *)k_fused_step_o4<R, VAR, false>)
                         : HYDRO_PICK(k_fused_step, R, VAR, f.rare, f.scheme);
-    return hipLaunchKernel(k, dim3(div_up(f.C, block)), dim3(block), args, step_lds<R>(f.scheme, block), s);
+    return hipLaunchKernel(k, dim3(f.ntiles), dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
 template <class R, int VAR>
@@ -637,7 +747,18 @@ hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock,
                         ? (f.rare ? (const void *)k_fused_step_tick_o4<R, VAR, true>
                                   : (const void *)k_fused_step_tick_o4<R, VAR, false>)
                         : HYDRO_PICK(k_fused_step_tick, R, VAR, f.rare, f.scheme);
-    return hipLaunchKernel(k, dim3(div_up(f.C, block)), dim3(block), args, step_lds<R>(f.scheme, block), s);
+    return hipLaunchKernel(k, dim3(f.ntiles), dim3(block), args, step_lds<R>(f.scheme, block), s);
+}
+
+__global__ void k_fill_i32(int32_t *p, int32_t v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(div_up(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_tick_advance(int64_t *tick, int by, hipStream_t s) {
@@ -655,14 +776,22 @@ hipError_t launch_clock_set(StepClock *clock, int day, int kt, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <class R, int VAR> const void *persistent_kernel(bool rare, int scheme, bool occ4) {
+    if (scheme == 0 && occ4)
+        return rare ? (const void *)k_persistent_tiles_o4<R, VAR, true> : (const void *)k_persistent_tiles_o4<R, VAR, false>;
+    if (scheme == 0)
+        return rare ? (const void *)k_persistent_tiles<R, VAR, true> : (const void *)k_persistent_tiles<R, VAR, false>;
+    return HYDRO_PICK(k_persistent, R, VAR, rare, scheme);
+}
+
 template <class R, int VAR>
-hipError_t persistent_max_grid(int block, bool rare, int scheme, int *grid) {
+hipError_t persistent_max_grid(int block, bool rare, int scheme, bool occ4, int *grid) {
     int dev = 0, per_cu = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, HYDRO_PICK(k_persistent, R, VAR, rare, scheme), block,
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel<R, VAR>(rare, scheme, occ4), block,
                                                      step_lds<R>(scheme, block));
     if (e != hipSuccess) return e;
     *grid = per_cu * cus;
@@ -675,8 +804,9 @@ hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, in
     FusedView<R> a = f;
     PersistentArgs p = pa;
     void *args[] = {&a, &p};
-    return hipLaunchCooperativeKernel(HYDRO_PICK(k_persistent, R, VAR, f.rare != 0, f.scheme), dim3(grid),
-                                      dim3(block), args, step_lds<R>(f.scheme, block), s);
+    return hipLaunchCooperativeKernel(persistent_kernel<R, VAR>(f.rare != 0, f.scheme, f.occ4 != 0), dim3(grid),
+                                      dim3(block), args,
+                                      step_lds<R>(f.scheme, block), s);
 }
 
 template <class R>
@@ -754,7 +884,7 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
                                                        int, int, hipStream_t);                          \
     template hipError_t launch_persistent<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, int, \
                                                   hipStream_t);                                         \
-    template hipError_t persistent_max_grid<R, VAR>(int, bool, int, int *);                                       \
+    template hipError_t persistent_max_grid<R, VAR>(int, bool, int, bool, int *);                                     \
     template hipError_t launch_batch_step<R, VAR>(const BatchView<R> &, int, int, int, hipStream_t);   \
     template hipError_t launch_batch_step_tick<R, VAR>(const BatchView<R> &, const int64_t *, int, int, int, \
                                                        hipStream_t);                                    \

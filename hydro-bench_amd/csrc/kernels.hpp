@@ -30,7 +30,8 @@ template <class R> struct FusedView {
     int C;
     int materialize;           // also write the caller's FLUX0..3 every step
     int scheme;                // hydro_scheme: 0 tiles (edge tasks in LDS), 1 one thread per cell
-    int block;                 // workgroup size = tile size (== blockDim.x of every launch)
+    int block;                 // workgroup size (== blockDim.x of every launch)
+    int tile;                  // cells per tile (<= block; == block for the CELL scheme)
     int ntiles;                // tiles = workgroups of a one-step launch (== gridDim.x)
     int occ4;                  // TILES scheme limited to 128 VGPRs (4 waves per SIMD)
     int rare;                  // mesh has KLAS 3/5/6/7 or unknown codes: launch the full variant
@@ -69,6 +70,11 @@ template <class R> struct FusedView {
     // state, and its flux sums WH, WU, WV as [3][C]
     State<R> *rec_state;
     R *rec_wsum;
+    // persistent TILES kernel: tiles sharing an edge with tile t are
+    // tile_nb[tile_nb_off[t] .. tile_nb_off[t+1]); tile_flag[t] = steps of
+    // the launch that tile t has completed
+    const int32_t *tile_nb_off, *tile_nb;
+    int32_t *tile_flag;
     // gauge read (null when off): workgroup 0 copies Z of the step's INPUT
     // state at the internal cells gauge_cells[0..gauge_n) to gauge_out
     const int32_t *gauge_cells;
@@ -110,6 +116,7 @@ hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, i
 template <class R, int VAR>
 hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock, int offset, int spd, int block,
                                   int xcd_remap, hipStream_t s);
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
 hipError_t launch_tick_advance(int64_t *tick, int by, hipStream_t s);
 hipError_t launch_clock_advance(StepClock *clock, int by, int spd, hipStream_t s);
 hipError_t launch_clock_set(StepClock *clock, int day, int kt, hipStream_t s);
@@ -125,7 +132,7 @@ struct PersistentArgs {
 template <class R, int VAR>
 hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, int block, int grid, hipStream_t s);
 template <class R, int VAR>
-hipError_t persistent_max_grid(int block, bool rare, int scheme, int *grid);
+hipError_t persistent_max_grid(int block, bool rare, int scheme, bool occ4, int *grid);
 
 // external H U V Z -> internal records (new order)
 template <class R>

@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define HYDRO_ABI_VERSION 1
+#define HYDRO_ABI_VERSION 2
 
 typedef enum hydro_status {
     HYDRO_OK = 0,
@@ -145,6 +145,9 @@ typedef struct hydro_tune {
     int32_t waves_per_simd; /* 0: compiler's choice; 4: fused kernels limited to 128 VGPRs so
                                4 waves fit per SIMD (one resident round at 207K cells) */
     int32_t scheme;         /* hydro_scheme */
+    int32_t tile_cells;     /* TILES scheme: cells per tile (64..block); 0 = block, or for a mesh
+                               with fewer tiles than CUs one tile per CU (at least 64 cells);
+                               batches use block */
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */

@@ -78,3 +78,26 @@ def test_batch_fp32_members_within_fp64_envelope():
     print(f"fp32 batch vs fp64 single runs, 100 steps: max |dH| = {worst:.3e} m")
     assert worst < 1e-3
     b.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_small_tiles_under_wide_workgroups(dtype):
+    """Meshes with fewer full tiles than CUs get tiles of fewer cells than
+    the workgroup has threads (here 64-79 cells under 256 threads): the
+    fused and persistent steps must still give the two-kernel bits (every
+    thread of the last tile loads task entries past its tile, which the
+    task arrays must hold)."""
+    import hydro_amd
+    from hydro_amd.device import DeviceMesh
+
+    tune = hydro_amd.default_tune(block=256)
+    for h in _members(dtype):
+        res = {}
+        for mode in ("two_kernel", "fused", "persistent"):
+            dm = DeviceMesh(h, device=0, tune=tune)
+            dm.step(50, mode)
+            res[mode] = dm.fetch("H", "U", "V", "Z")
+            dm.close()
+        for mode in ("fused", "persistent"):
+            for k in ("H", "U", "V", "Z"):
+                assert np.array_equal(res[mode][k], res["two_kernel"][k]), f"{h.CELL} cells {mode} {k}"

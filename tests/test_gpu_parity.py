@@ -215,3 +215,21 @@ def test_determinism_persistent():
         dm.close()
     for k in ("H", "U", "V"):
         assert np.array_equal(out[0][k], out[1][k])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("tune", [dict(), dict(waves_per_simd=4), dict(persistent_wgs_per_cu=1)],
+                         ids=["default", "occ4", "one-wg-per-cu"])
+def test_persistent_tiles_equal_fused_207k(dtype, tune):
+    """The persistent TILES kernel (tiles wait only on the tiles they share an
+    edge with) gives the fused launches' bits at full size after 300 steps,
+    also when every workgroup walks several tiles per step."""
+    host = cached_mesh("floodplain_207234", dtype=dtype)
+    res = {}
+    for mode in ("fused", "persistent"):
+        dm = _device_mesh(host, **tune)
+        dm.step(300, mode)
+        res[mode] = dm.fetch("H", "U", "V", "Z")
+        dm.close()
+    for k in ("H", "U", "V", "Z"):
+        assert np.array_equal(res["persistent"][k].view(np.uint8), res["fused"][k].view(np.uint8)), k
