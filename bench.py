@@ -152,7 +152,7 @@ def ensemble_bench(a):
     import hydro_amd
     import hydro_amd.mesh as M
     from hydro_amd.device import DeviceMesh
-    from hydro_amd.ensemble import Batch
+    from hydro_amd.ensemble import Batch, autotune_batch
 
     steps = 500
     t0 = time.perf_counter()
@@ -186,7 +186,10 @@ def ensemble_bench(a):
             dm.sync()
         results[mode] = [dm.fetch("H", "U", "V", "Z") for dm in dms]
         _log(f"sequential {mode}: {times[f'sequential_{mode}']:.1f} ms")
-    b = Batch(dms, tune=tune)
+    # the batch's workgroup size and mode: chosen by timing 20 steps of each
+    (block, auto_mode), table = autotune_batch(dms, steps=20)
+    _log(f"autotune: block {block}, {auto_mode} ({', '.join(f'{k[0]}/{k[1]} {v:.0f} us' for k, v in table.items())})")
+    b = Batch(dms, tune=hydro_amd.default_tune(block=block))
     same = {}
     for mode in ("fused", "graph", "persistent"):
         for rep in range(2):
@@ -221,7 +224,8 @@ def ensemble_bench(a):
         "n_gpus": 1, "steps": steps, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (128 seeded floodplain meshes, seed 3003, cells log-uniform in [2000, 207234])",
         "config": {"workload": "ensemble_128", "members": len(cells), "cells": sum(cells), "min_cells": min(cells),
-                   "max_cells": max(cells), "block": a.block, "batched_mode": best},
+                   "max_cells": max(cells), "block": block, "batched_mode": best, "autotuned_mode": auto_mode},
+        "autotune_us_per_step": {f"{k[0]}/{k[1]}": v for k, v in table.items()},
         "speedup_vs_sequential_fused": times["sequential_fused"] / times[f"batched_{best}"],
         "ms": times,
         "cell_steps_per_s": {k: total / (v / 1e3) for k, v in times.items()},

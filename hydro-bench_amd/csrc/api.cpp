@@ -858,6 +858,117 @@ int hydro_sync(hydro_mesh *m, hipStream_t stream) {
     return (m->d.dtype == HYDRO_F64) ? ensure_external<double>(m, stream) : ensure_external<float>(m, stream);
 }
 
+// ---- autotuner ---------------------------------------------------------------
+
+namespace {
+// Copies of the caller's mutable arrays (state, W, KLAS, FLUX0..3), so every
+// candidate starts from the same step.
+struct Snapshot {
+    DevBuf buf[10];
+    void *src[10] = {};
+    size_t bytes[10] = {};
+    int n = 0;
+    int take(hydro_mesh *m, hipStream_t s) {
+        const size_t rs = m->d.dtype == HYDRO_F64 ? 8 : 4, C = (size_t)m->CELL;
+        void *p[10] = {m->d.H, m->d.U, m->d.V, m->d.Z, m->d.W, m->d.KLAS, m->d.FLUX0, m->d.FLUX1, m->d.FLUX2,
+                       m->d.FLUX3};
+        const size_t b[10] = {C * rs, C * rs, C * rs, C * rs, C * rs, 4 * C * 4, 4 * C * rs, 4 * C * rs, 4 * C * rs,
+                              4 * C * rs};
+        for (int i = 0; i < 10; i++) {
+            if (!p[i]) continue;
+            int rc = buf[n].alloc(b[i]);
+            if (rc) return rc;
+            src[n] = p[i];
+            bytes[n] = b[i];
+            HIP_TRY(hipMemcpyAsync(buf[n].p, p[i], b[i], hipMemcpyDeviceToDevice, s));
+            n++;
+        }
+        return HYDRO_OK;
+    }
+    int put(hipStream_t s) const {
+        for (int i = 0; i < n; i++) HIP_TRY(hipMemcpyAsync(src[i], buf[i].p, bytes[i], hipMemcpyDeviceToDevice, s));
+        return HYDRO_OK;
+    }
+};
+
+// Back to the snapshot's state at (day, kt), in the mesh's current layout.
+int restore_snapshot(hydro_mesh *m, const Snapshot &snap, int day, int kt, hipStream_t s) {
+    int rc = (m->d.dtype == HYDRO_F64) ? ensure_external<double>(m, s) : ensure_external<float>(m, s);
+    if (rc || (rc = snap.put(s))) return rc;
+    m->day = day;
+    m->kt = kt;
+    m->ext_valid = true;
+    m->int_valid = false;
+    if (!m->prepared) return HYDRO_OK;
+    return (m->d.dtype == HYDRO_F64) ? ensure_internal<double>(m, s) : ensure_internal<float>(m, s);
+}
+}  // namespace
+
+int hydro_autotune(hydro_mesh *m, int mode, int steps, const hydro_tune *cands, int ncand, hydro_tune *best,
+                   float *us_per_step, hipStream_t s) {
+    if (!m || !best || steps <= 0 || ncand < 0 || (ncand > 0 && !cands)) return fail(HYDRO_ERR_INVALID, "bad arguments");
+    if (mode != HYDRO_MODE_FUSED && mode != HYDRO_MODE_GRAPH && mode != HYDRO_MODE_PERSISTENT)
+        return fail(HYDRO_ERR_UNSUPPORTED, "autotune times the FUSED, GRAPH or PERSISTENT mode");
+    std::vector<hydro_tune> c;
+    if (ncand > 0) {
+        c.assign(cands, cands + ncand);
+    } else {
+        const hydro_tune d = default_tune();
+        auto add = [&](int block, int tile, int scheme, int wps) {
+            hydro_tune t = d;
+            t.block = block;
+            t.tile_cells = tile;
+            t.scheme = scheme;
+            t.waves_per_simd = wps;
+            c.push_back(t);
+        };
+        add(256, 0, HYDRO_SCHEME_TILES, 0);
+        add(256, 128, HYDRO_SCHEME_TILES, 0);
+        add(128, 0, HYDRO_SCHEME_TILES, 0);
+        add(256, 0, HYDRO_SCHEME_TILES, 4);
+        add(128, 0, HYDRO_SCHEME_CELL, 0);
+        add(256, 0, HYDRO_SCHEME_CELL, 0);
+    }
+    const int per = m->spd - 1;
+    const int64_t remaining = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
+    const int warm = std::min(steps, 5);
+    if (remaining < steps) return fail(HYDRO_ERR_RANGE, "%d steps requested, %lld remain", steps, (long long)remaining);
+    int rc = (m->d.dtype == HYDRO_F64) ? ensure_external<double>(m, s) : ensure_external<float>(m, s);
+    if (rc) return rc;
+    Snapshot snap;
+    if ((rc = snap.take(m, s))) return rc;
+    const int day0 = m->day, kt0 = m->kt;
+    const hydro_tune before = m->tune;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best_us = -1;
+    int best_i = -1;
+    for (size_t i = 0; i < c.size(); i++) {
+        float us = -1;
+        if (hydro_prepare(m, &c[i]) == HYDRO_OK && restore_snapshot(m, snap, day0, kt0, s) == HYDRO_OK &&
+            hydro_step_n(m, warm, mode, s) == HYDRO_OK && restore_snapshot(m, snap, day0, kt0, s) == HYDRO_OK &&
+            hipEventRecord(e0, s) == hipSuccess && hydro_step_n(m, steps, mode, s) == HYDRO_OK &&
+            hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) us = ms * 1000.0f / (float)steps;
+        }
+        if (us_per_step) us_per_step[i] = us;
+        if (us > 0 && (best_i < 0 || us < best_us)) {
+            best_us = us;
+            best_i = (int)i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const hydro_tune chosen = best_i >= 0 ? c[best_i] : before;
+    if ((rc = hydro_prepare(m, &chosen)) || (rc = restore_snapshot(m, snap, day0, kt0, s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    *best = chosen;
+    if (best_i < 0) return fail(HYDRO_ERR_UNSUPPORTED, "no candidate tuning ran");
+    return HYDRO_OK;
+}
+
 int hydro_reload(hydro_mesh *m, hipStream_t stream) {
     if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
     m->ext_valid = true;

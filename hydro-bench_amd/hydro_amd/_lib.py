@@ -71,6 +71,13 @@ class Tune(ctypes.Structure):
     ]
 
 
+def autotune_candidates():
+    """The library's built-in candidate list of hydro_autotune."""
+    return [default_tune(block=256, tile_cells=0), default_tune(block=256, tile_cells=128),
+            default_tune(block=128, tile_cells=0), default_tune(block=256, tile_cells=0, waves_per_simd=4),
+            default_tune(block=128, scheme=SCHEME_CELL), default_tune(block=256, scheme=SCHEME_CELL)]
+
+
 def default_tune(**kw) -> Tune:
     t = Tune(block=256, reorder=REORDER_TILES, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
              materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0)
@@ -114,6 +121,7 @@ SIGNATURES = {
     "hydro_cells_map": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "hydro_states_map": (c_int, [c_void_p] * 6 + [c_int, c_void_p]),
     "hydro_refresh_statics": (c_int, [c_void_p, c_void_p]),
+    "hydro_autotune": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, POINTER(Tune), c_void_p, c_void_p]),
     "hydro_misfit_forward": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "hydro_misfit_backward": (c_int, [c_void_p, c_int, c_int] + [c_void_p] * 2 + [c_int] + [c_void_p] * 6
                               + [POINTER(c_int), c_void_p]),

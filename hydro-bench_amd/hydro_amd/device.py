@@ -87,6 +87,20 @@ class DeviceMesh:
         _lib.check(self.lib.hydro_prepare(self.h, ctypes.byref(t)))
 
     # -- stream -------------------------------------------------------------
+    def autotune(self, mode: str = "fused", steps: int = 50, candidates=None):
+        """Time candidate tunings of the reordered path (hydro_autotune) and
+        keep the fastest; the state and time are left as they were.  Returns
+        (best Tune, [(Tune, us per step), ...]); -1 marks a candidate that
+        could not run."""
+        cands = list(candidates) if candidates else _lib.autotune_candidates()
+        arr = (_lib.Tune * len(cands))(*cands)
+        times = (ctypes.c_float * len(cands))()
+        best = _lib.Tune()
+        _lib.check(self.lib.hydro_autotune(self.h, _lib.MODES[mode], int(steps), arr, len(cands), ctypes.byref(best),
+                                           times, self.stream()))
+        self.tune = best
+        return best, list(zip(cands, list(times)))
+
     def stream(self):
         return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
 

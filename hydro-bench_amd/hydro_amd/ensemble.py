@@ -47,3 +47,40 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def autotune_batch(members: List[DeviceMesh], steps: int = 20, blocks=(256, 128),
+                   modes=("fused", "graph", "persistent")):
+    """Choose the workgroup size and execution mode of a batch by timing
+    `steps` steps of each combination from the members' current states
+    (restored after every trial).  Returns ((block, mode), {(block, mode):
+    us per step}); the members are left prepared for the last block tried
+    and at their original states and times."""
+    import torch
+
+    saved = [(m.fetch("H", "U", "V", "Z"), m.get_time()) for m in members]
+
+    def restore():
+        for m, (st, (day, kt, _)) in zip(members, saved):
+            m.load_state(st)
+            m.set_time(day, kt)
+
+    table = {}
+    for block in blocks:
+        b = Batch(members, _lib.default_tune(block=block))
+        for mode in modes:
+            restore()
+            b.step(0, mode)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            b.step(steps, mode)
+            e1.record()
+            torch.cuda.synchronize()
+            table[(block, mode)] = e0.elapsed_time(e1) * 1e3 / steps
+        b.sync()
+        b.close()
+    restore()
+    best = min(table, key=table.get)
+    return best, table

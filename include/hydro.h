@@ -163,6 +163,19 @@ int hydro_mesh_destroy(hydro_mesh *m);
  * tune may be NULL for defaults. */
 int hydro_prepare(hydro_mesh *m, const hydro_tune *tune);
 
+/* Autotuner (no counterpart in the reference): times each candidate tuning of
+ * the reordered path on this mesh (prepare, up to 5 warm-up steps, then
+ * `steps` timed steps in `mode` = FUSED, GRAPH or PERSISTENT, every candidate
+ * from the same state and time), then leaves the mesh prepared with the
+ * fastest, its state and time as they were, and writes that tuning to *best.
+ * cands == NULL with ncand 0 tries a built-in list over block (128, 256),
+ * cells per tile, the 128-VGPR build and both schemes; us_per_step
+ * (optional, one entry per candidate) receives each time, -1 for a
+ * candidate that could not run.  Needs `steps` steps left before the NDAYS
+ * cap; synchronises the stream. */
+int hydro_autotune(hydro_mesh *m, int mode, int steps, const hydro_tune *cands, int ncand, hydro_tune *best,
+                   float *us_per_step, hipStream_t stream);
+
 /* ---- the reference operator API ----------------------------------------- */
 
 /* One flux pass: replaces CalculateFlux(mesh, kt, jt, stream)
