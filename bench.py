@@ -328,10 +328,10 @@ def calibrate_bench(a):
     for h in range(a.hypotheses):
         cal = calibrate(prob, t(field(6000 + h)), zbc, iters=a.iters)
         err = float(np.sqrt(np.mean((cal.n.cpu().numpy() - truth) ** 2)))
-        res.append({"hypothesis": h, "misfit_first": cal.losses[0], "misfit_final": cal.losses[-1],
+        res.append({"hypothesis": h, "misfit_first": cal.losses[0], "misfit_final": cal.best,
                     "rms_n_error_first": float(np.sqrt(np.mean((field(6000 + h) - truth) ** 2))),
                     "rms_n_error_final": err})
-        _log(f"hypothesis {h}: misfit {cal.losses[0]:.3e} -> {cal.losses[-1]:.3e}")
+        _log(f"hypothesis {h}: misfit {cal.losses[0]:.3e} -> {cal.best:.3e}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ratios = [r["misfit_final"] / r["misfit_first"] for r in res]

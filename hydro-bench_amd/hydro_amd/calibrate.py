@@ -32,8 +32,9 @@ def smooth_field(x, y, seed, mean, std, length, n_modes=24):
 
 @dataclass
 class Calibration:
-    n: "object"                               # final Manning field (torch f64, caller order)
-    losses: List[float] = field(default_factory=list)   # misfit before each update, then the final one
+    n: "object"                               # best Manning field seen (torch f64, caller order)
+    losses: List[float] = field(default_factory=list)   # misfit of every evaluated field, in order
+    best: float = float("inf")                # misfit of n
 
 
 def misfit_and_gradient(prob: MisfitProblem, n, zbc):
@@ -44,29 +45,44 @@ def misfit_and_gradient(prob: MisfitProblem, n, zbc):
 
 
 def calibrate(prob: MisfitProblem, n0, zbc, iters: int = 20, n_min: float = 0.005, n_max: float = 0.2,
-              first_step: float = 2e-3) -> Calibration:
+              first_step: float = 2e-3, max_growth: float = 4.0) -> Calibration:
     """`iters` gradient steps from n0; the first moves the largest entry by
     `first_step`, later ones use the Barzilai-Borwein length
     (dn.dn)/(dn.dg) of the last two iterates (the previous length when that
-    is not positive)."""
+    is not positive), at most `max_growth` times the previous length.  A
+    step that raises the misfit is undone and the length quartered.  The
+    result is the best field seen; `losses` lists the misfit of every
+    evaluated field."""
     n = n0.clone()
     out = Calibration(n=n)
     prev_n = prev_g = None
     alpha = None
+    best_loss, best_n, best_g = None, None, None
     for _ in range(iters):
         loss, g = misfit_and_gradient(prob, n, zbc)
         out.losses.append(loss)
-        if prev_g is None:
+        if best_loss is not None and loss > best_loss:
+            # back to the best field with a shorter step along its gradient
+            alpha *= 0.25
+            n = (best_n - alpha * best_g).clamp(n_min, n_max)
+            prev_n = prev_g = None
+            continue
+        best_loss, best_n, best_g = loss, n.clone(), g
+        if alpha is None:
             alpha = first_step / max(float(g.abs().max()), 1e-300)
-        else:
+        elif prev_g is not None:
             dn = n - prev_n
             dg = g - prev_g
             den = float((dn * dg).sum())
             if den > 0:
-                alpha = float((dn * dn).sum()) / den
+                alpha = min(float((dn * dn).sum()) / den, max_growth * alpha)
         prev_n, prev_g = n.clone(), g
         n = (n - alpha * g).clamp(n_min, n_max)
     zg, _ = run_forward(prob, n, zbc, keep_checkpoints=False)
-    out.losses.append(float(((zg - prob.z_obs) ** 2).sum()))
-    out.n = n
+    last = float(((zg - prob.z_obs) ** 2).sum())
+    out.losses.append(last)
+    if last <= best_loss:
+        best_loss, best_n = last, n
+    out.n = best_n
+    out.best = best_loss
     return out
