@@ -252,6 +252,9 @@ def _finish(name, raw, dtype, lib=None) -> HostMesh:
         v, d = _series(series, K0, R)
         ZT[np.arange(NDAYS) * C + cell] = v
         DZT[np.arange(NDAYS) * C + cell] = d
+    for k in ("QT", "DQT", "ZT", "DZT"):   # series already laid out by the caller
+        if k in raw:
+            {"QT": QT, "DQT": DQT, "ZT": ZT, "DZT": DZT}[k][:] = raw[k].astype(dtype)
     NHQ = int(raw.get("NHQ", 0))
     arrays = dict(
         H=outc["H"], U=raw.get("U0", np.zeros(C)).astype(dtype), V=raw.get("V0", np.zeros(C)).astype(dtype),
