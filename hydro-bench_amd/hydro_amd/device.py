@@ -50,7 +50,7 @@ class DeviceMesh:
         self.variant = variant
         self.t = {}
         for k, v in host.arrays.items():
-            if k in ("NAP", "XP", "YP"):
+            if k in ("NAP", "XP", "YP", "XYMIN"):
                 continue
             self.t[k] = torch.from_numpy(np.ascontiguousarray(v)).to(self.device)
         d = _lib.MeshDesc()

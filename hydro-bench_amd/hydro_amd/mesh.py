@@ -226,6 +226,7 @@ def _finish(name, raw, dtype, lib=None) -> HostMesh:
     C, NOD = raw["CELL"], raw["NOD"]
     XP = raw["XP"].astype(dtype)
     YP = raw["YP"].astype(dtype)
+    XYMIN = np.array([XP.min(), YP.min()], dtype=np.float64)   # preCalculate shifts XP, YP by these
     ZBC = raw["ZBC"].astype(dtype)
     Z = raw["Zinit"].astype(dtype)
     FNC0 = raw["n"].astype(dtype)
@@ -268,7 +269,7 @@ def _finish(name, raw, dtype, lib=None) -> HostMesh:
         SIDE=out["SIDE"], COSF=out["COSF"], SINF=out["SINF"], SLCOS=out["SLCOS"], SLSIN=out["SLSIN"],
         FLUX0=np.zeros(4 * C, dtype=dtype), FLUX1=np.zeros(4 * C, dtype=dtype),
         FLUX2=np.zeros(4 * C, dtype=dtype), FLUX3=np.zeros(4 * C, dtype=dtype),
-        NAP=NAP, XP=XP, YP=YP,
+        NAP=NAP, XP=XP, YP=YP, XYMIN=XYMIN,
     )
     return HostMesh(name=name, dtype=dtype, CELL=C, NDAYS=NDAYS, NHQ=NHQ, MDT=int(raw["MDT"]),
                     DT=float(raw["DT"]), HM1=float(raw["HM1"]), HM2=float(raw["HM2"]), JL=float(raw.get("JL", 0.0)),
