@@ -309,15 +309,28 @@ template <class R> int prepare_t(hydro_mesh *m) {
     // TILES scheme tasks.  An interior edge whose two slots lie in one tile,
     // one primary and one mirror, with exactly opposite normals, is one pair
     // task (listed from its primary slot); every other non-padding slot is a
-    // one-slot task.  Within a tile: pairs, then interior slots, then
-    // boundary slots, so most waves run one code path.  Every tile gets the
-    // same number of entries (the largest task count, rounded up to a
-    // wavefront), so a tile's entries start at t * stride.
+    // one-slot task.
+    //
+    // Layout: thread i of the workgroup runs entries i, i + block, i + 2 block,
+    // ... of its tile, so each wavefront runs a column of 64-entry chunks.  A
+    // wavefront pays for every code path any of its lanes takes, so chunks
+    // are homogeneous: pair chunks, and one-slot chunks (interior slots, then
+    // boundary slots).  The leftover pairs of a tile (its pair count modulo
+    // 64) become two one-slot tasks each when they fit in the free entries
+    // of the last one-slot chunk, which costs nothing there and saves a
+    // mostly empty pair chunk.  Chunks go to the wavefront with the least
+    // work so far, heaviest first (a one-slot chunk counts 3/2 of a pair
+    // chunk), so the wavefronts of a workgroup, which run on different
+    // SIMDs, carry near-equal work.  Every tile gets the same number of
+    // entries (its rounds times the block, the largest over tiles), so a
+    // tile's entries start at t * stride.
     {
         const int B = m->tile;
         const int ntiles = (C + B - 1) / B;
         std::vector<std::vector<uint32_t>> lists(ntiles);
         std::vector<uint8_t> pad_n(C, 0);
+        constexpr size_t kWave = 64;
+        const int nwave = std::max(1, m->tune.block / (int)kWave);
         int npairs = 0, nsingles = 0;
         size_t most = 0;
         auto neg_equal = [](R a, R b) {
@@ -362,14 +375,54 @@ template <class R> int prepare_t(hydro_mesh *m) {
                     }
                     sgl.push_back(task_word(p - p0, j, 0, 0, false, kind));
                 }
+            // leftover pairs -> one-slot tasks when the last one-slot chunk has room
+            const size_t left = pairs.size() % kWave, ones = sgl.size() + bnd.size();
+            const size_t room = (ones + kWave - 1) / kWave * kWave - ones;
+            if (left > 0 && 2 * left <= room) {
+                std::vector<uint32_t> split;
+                for (size_t q = pairs.size() - left; q < pairs.size(); q++) {
+                    const uint32_t w = pairs[q];
+                    split.push_back(task_word(w & 0xff, (w >> 8) & 3, 0, 0, false, K_PRIMARY));
+                    split.push_back(task_word((w >> 10) & 0xff, (w >> 18) & 3, 0, 0, false, K_MIRROR));
+                }
+                pairs.resize(pairs.size() - left);
+                npairs -= (int)left;
+                sgl.insert(sgl.begin(), split.begin(), split.end());
+            }
+            std::vector<uint32_t> one(sgl);
+            one.insert(one.end(), bnd.begin(), bnd.end());
+            nsingles += (int)one.size();
+            // chunks: {list, first entry, entries, cost}
+            struct Chunk {
+                const std::vector<uint32_t> *src;
+                size_t at, n;
+                int cost;
+            };
+            std::vector<Chunk> ch;
+            for (size_t q = 0; q < pairs.size(); q += kWave)
+                ch.push_back({&pairs, q, std::min<size_t>(kWave, pairs.size() - q), 2});
+            for (size_t q = 0; q < one.size(); q += kWave)
+                ch.push_back({&one, q, std::min<size_t>(kWave, one.size() - q), 3});
+            std::stable_sort(ch.begin(), ch.end(), [](const Chunk &a, const Chunk &b) { return a.cost > b.cost; });
+            std::vector<int> load(nwave, 0), cnt(nwave, 0);
+            std::vector<std::pair<int, const Chunk *>> placed;
+            for (const Chunk &c : ch) {
+                int w = 0;
+                for (int v = 1; v < nwave; v++)
+                    if (load[v] < load[w] || (load[v] == load[w] && cnt[v] < cnt[w])) w = v;
+                placed.push_back({cnt[w] * nwave + w, &c});
+                load[w] += c.cost;
+                cnt[w]++;
+            }
+            const int rounds = std::max(1, *std::max_element(cnt.begin(), cnt.end()));
             std::vector<uint32_t> &L = lists[t];
-            L.insert(L.end(), pairs.begin(), pairs.end());
-            L.insert(L.end(), sgl.begin(), sgl.end());
-            L.insert(L.end(), bnd.begin(), bnd.end());
-            nsingles += (int)(sgl.size() + bnd.size());
+            L.assign((size_t)rounds * nwave * kWave, TASK_NONE);
+            for (const auto &pc : placed)
+                std::copy_n(pc.second->src->begin() + pc.second->at, pc.second->n,
+                            L.begin() + (size_t)pc.first * kWave);
             most = std::max(most, L.size());
         }
-        const size_t stride = std::max<size_t>(64, (most + 63) / 64 * 64);
+        const size_t stride = std::max<size_t>(64, most);
         // spare entries: every thread of the last tile's workgroup loads
         // entries k0 + i and k0 + i + block unconditionally (block threads,
         // which may be more than the tile's cells)
@@ -381,6 +434,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
             const int p0 = t * B;
             for (size_t q = 0; q < lists[t].size(); q++) {
                 const uint32_t w = lists[t][q];
+                if (w == TASK_NONE) continue;
                 const size_t k = (size_t)t * stride + q;
                 const int pa = p0 + (int)(w & 0xff), ja = (int)((w >> 8) & 3);
                 const size_t ea = (size_t)ja * C + pa;
