@@ -108,7 +108,7 @@ struct hydro_mesh {
     int cur = 0;  // state record buffer holding the current state
     bool ext_valid = true, int_valid = false;
     DevBuf st[2], ZBC, AREA, FNC, DTA, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
-    DevBuf task_w, task_nb, task_cs, task_sd, cell_pad;   // TILES scheme task entries
+    DevBuf task_w, task_nb, task_cs, task_sd, task_n, cell_pad;   // TILES scheme task entries
     DevBuf tile_nb_off, tile_nb, tile_flag;   // persistent TILES kernel: neighbour tiles (CSR), step flags
     int max_tile_nb = 0;
     int tile = 256;                 // cells per tile (resolved hydro_tune.tile_cells)
@@ -188,6 +188,7 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.task_nb = m->task_nb.as<int32_t>();
     f.task_cs = m->task_cs.as<R>();
     f.task_sd = m->task_sd.as<R>();
+    f.task_n = m->task_n.as<int32_t>();
     f.cell_pad = m->cell_pad.as<uint8_t>();
     f.materialize = m->tune.materialize_flux;
     f.scheme = m->tune.scheme;
@@ -331,6 +332,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
         std::vector<uint8_t> pad_n(C, 0);
         constexpr size_t kWave = 64;
         const int nwave = std::max(1, m->tune.block / (int)kWave);
+        std::vector<int32_t> used_n(ntiles, 0);   // 64-entry positions a tile uses
         int npairs = 0, nsingles = 0;
         size_t most = 0;
         auto neg_equal = [](R a, R b) {
@@ -364,7 +366,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
                         const bool pairable = twin_n[sk] == j && nbr_n[sk] == p &&
                                               ((kind == K_PRIMARY && kn == K_MIRROR) ||
                                                (kind == K_MIRROR && kn == K_PRIMARY)) &&
-                                              neg_equal(cos_n[sn], cos_n[sk]) && neg_equal(sin_n[sn], sin_n[sk]);
+                                              neg_equal(cos_n[sn], cos_n[sk]) && neg_equal(sin_n[sn], sin_n[sk]) &&
+                                              std::memcmp(&side_n[sn], &side_n[sk], sizeof(R)) == 0;
                         if (pairable) {
                             if (kind == K_PRIMARY) {
                                 pairs.push_back(task_word(p - p0, j, n - p0, k, true, 0));
@@ -405,21 +408,31 @@ template <class R> int prepare_t(hydro_mesh *m) {
                 ch.push_back({&one, q, std::min<size_t>(kWave, one.size() - q), 3});
             std::stable_sort(ch.begin(), ch.end(), [](const Chunk &a, const Chunk &b) { return a.cost > b.cost; });
             std::vector<int> load(nwave, 0), cnt(nwave, 0);
-            std::vector<std::pair<int, const Chunk *>> placed;
+            std::vector<std::pair<int, const Chunk *>> placed;   // (wavefront, chunk)
             for (const Chunk &c : ch) {
                 int w = 0;
                 for (int v = 1; v < nwave; v++)
                     if (load[v] < load[w] || (load[v] == load[w] && cnt[v] < cnt[w])) w = v;
-                placed.push_back({cnt[w] * nwave + w, &c});
+                placed.push_back({w, &c});
                 load[w] += c.cost;
                 cnt[w]++;
             }
-            const int rounds = std::max(1, *std::max_element(cnt.begin(), cnt.end()));
+            // wavefronts with more chunks first, so the unused positions are
+            // the last ones and a tile's entries end at its used positions
+            std::vector<int> by(nwave), slot(nwave), next(nwave, 0);
+            for (int v = 0; v < nwave; v++) by[v] = v;
+            std::stable_sort(by.begin(), by.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+            for (int v = 0; v < nwave; v++) slot[by[v]] = v;
+            int used = 0;
             std::vector<uint32_t> &L = lists[t];
-            L.assign((size_t)rounds * nwave * kWave, TASK_NONE);
-            for (const auto &pc : placed)
-                std::copy_n(pc.second->src->begin() + pc.second->at, pc.second->n,
-                            L.begin() + (size_t)pc.first * kWave);
+            L.clear();
+            for (const auto &pc : placed) {
+                const int w = pc.first, pos = next[w]++ * nwave + slot[w];
+                used = std::max(used, pos + 1);
+                if (L.size() < (size_t)(pos + 1) * kWave) L.resize((size_t)(pos + 1) * kWave, TASK_NONE);
+                std::copy_n(pc.second->src->begin() + pc.second->at, pc.second->n, L.begin() + (size_t)pos * kWave);
+            }
+            used_n[t] = used;
             most = std::max(most, L.size());
         }
         const size_t stride = std::max<size_t>(64, most);
@@ -429,7 +442,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
         const size_t ntask = stride * (size_t)ntiles + 2 * (size_t)m->tune.block;
         std::vector<uint32_t> tw(ntask, TASK_NONE);
         std::vector<int32_t> tnb(ntask, -1);
-        std::vector<R> tcs(2 * ntask, (R)0), tsd(2 * ntask, (R)0);
+        std::vector<R> tcs(2 * ntask, (R)0), tsd(ntask, (R)0);
         for (int t = 0; t < ntiles; t++) {
             const int p0 = t * B;
             for (size_t q = 0; q < lists[t].size(); q++) {
@@ -442,18 +455,14 @@ template <class R> int prepare_t(hydro_mesh *m) {
                 tnb[k] = nbr_n[ea];
                 tcs[2 * k] = cos_n[ea];
                 tcs[2 * k + 1] = sin_n[ea];
-                tsd[2 * k] = side_n[ea];
-                if (w & TASK_PAIR) {
-                    const int pb = p0 + (int)((w >> 10) & 0xff), jb = (int)((w >> 18) & 3);
-                    tsd[2 * k + 1] = side_n[(size_t)jb * C + pb];
-                }
+                tsd[k] = side_n[ea];   // a pair's two slots have the same SIDE (a pairing condition)
             }
         }
         m->n_pairs = npairs;
         m->n_singles = nsingles;
         m->task_stride = (int)stride;
         if ((rc = h2d(m->task_w, tw)) || (rc = h2d(m->task_nb, tnb)) || (rc = h2d(m->task_cs, tcs)) ||
-            (rc = h2d(m->task_sd, tsd)) || (rc = h2d(m->cell_pad, pad_n)))
+            (rc = h2d(m->task_sd, tsd)) || (rc = h2d(m->task_n, used_n)) || (rc = h2d(m->cell_pad, pad_n)))
             return rc;
     }
     {

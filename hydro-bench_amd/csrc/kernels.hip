@@ -250,14 +250,22 @@ __device__ __forceinline__ void task_neighbour(const FusedView<R> &f, const Stat
     t.zbn = f.ZBC[ni];
 }
 
+template <class R> __device__ __forceinline__ TaskIn<R> none_task() {
+    TaskIn<R> t;
+    t.w = TASK_NONE;
+    t.nb = -1;
+    t.c = t.s = t.sa = t.sb = 0;
+    return t;
+}
+
 template <class R> __device__ __forceinline__ TaskIn<R> task_load(const FusedView<R> &f, int k) {
     TaskIn<R> t;
     t.w = f.task_w[k];
     t.nb = f.task_nb[k];
     t.c = f.task_cs[2 * k];
     t.s = f.task_cs[2 * k + 1];
-    t.sa = f.task_sd[2 * k];
-    t.sb = f.task_sd[2 * k + 1];
+    t.sa = f.task_sd[k];
+    t.sb = t.sa;
     return t;
 }
 
@@ -345,13 +353,14 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
 #pragma unroll
         for (int j = 0; j < 4; j++) sd[j] = f.SIDE[j * C + pc];
     }
-    // the task arrays carry 2*block spare entries after the last tile, so
-    // both loads are unconditional; entries past this tile's stride are
-    // dropped by a select (no branch around the loads)
-    const int k0 = t * f.task_stride, kn = k0 + f.task_stride;
-    TaskIn<R> t0 = task_load<R>(f, k0 + i), t1 = task_load<R>(f, k0 + i + B);
-    if (k0 + i >= kn) t0.w = TASK_NONE;
-    if (k0 + i + B >= kn) t1.w = TASK_NONE;
+    // a wavefront reads the entries of its first two positions when the tile
+    // uses them (the test is uniform over the wavefront: positions are 64
+    // entries); the positions a tile leaves unused are not read
+    const int k0 = t * f.task_stride, kn = k0 + 64 * f.task_n[t];
+    const int kw = k0 + (i & ~63);
+    TaskIn<R> t0 = none_task<R>(), t1 = none_task<R>();
+    if (kw < kn) t0 = task_load<R>(f, k0 + i);
+    if (kw + B < kn) t1 = task_load<R>(f, k0 + i + B);
     task_neighbour<COH>(f, Sin, t0, pc);
     task_neighbour<COH>(f, Sin, t1, pc);
     if (live) {

@@ -58,13 +58,16 @@ template <class R> struct FusedView {
     //   task_w   packed word (TaskWord), TASK_NONE for unused entries
     //   task_nb  new-order neighbour of a one-slot task (-1 none)
     //   task_cs  [2k] COSF, SINF of the (primary) slot
-    //   task_sd  [2k] SIDE of the slot and of the mirror slot of a pair
-    // The arrays hold 2*block TASK_NONE entries past the last tile.
+    //   task_sd  [k] SIDE of the slot (both slots of a pair have the same)
+    // task_n[t]: 64-entry positions tile t uses (the rest hold TASK_NONE and
+    // are not read).  The arrays hold 2*block TASK_NONE entries past the last
+    // tile.
     // cell_pad[p]: bit j set when slot j of cell p is a triangle's padding slot.
     int task_stride;
     const uint32_t *task_w;
     const int32_t *task_nb;
     const R *task_cs, *task_sd;
+    const int32_t *task_n;
     const uint8_t *cell_pad;
     // recording for the adjoint (null when not recording): the step's input
     // state, and its flux sums WH, WU, WV as [3][C]
