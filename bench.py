@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=900)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--mode", default="fused", choices=["persistent", "graph", "fused", "two_kernel"])
+    ap.add_argument("--mode", default="fused", choices=["persistent", "graph", "fused", "two_kernel", "graph_two_kernel"])
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -102,7 +102,8 @@ def time_gpu(host, mode, steps, warmup, block):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
-    launches = {"persistent": 1, "graph": steps, "fused": steps, "two_kernel": 2 * steps}[mode]
+    launches = {"persistent": 1, "graph": steps, "fused": steps, "two_kernel": 2 * steps,
+                "graph_two_kernel": 2 * steps}[mode]
     st = dm.fetch("H")
     dm.close()
     return ms, wall, launches, st
@@ -383,13 +384,15 @@ def main():
     C = host64.CELL
     value = C * steps / (ms / 1e3)
     per_launch_ms = ms / launches
-    steps_per_launch = steps / launches if a.mode != "two_kernel" else 1.0
+    two = a.mode in ("two_kernel", "graph_two_kernel")
+    steps_per_launch = steps / launches if not two else 1.0
     bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C * (steps if a.mode == "persistent" else 1)
-    if a.mode == "two_kernel":
+    if two:
         bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C / 2.0
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
     kernel = {"persistent": "k_persistent<double,0,false,0>", "graph": "k_fused_step_tick<double,0,false,0>",
-              "fused": "k_fused_step<double,0,false,0>", "two_kernel": "k_calculate_flux+k_update_cell"}[a.mode]
+              "fused": "k_fused_step<double,0,false,0>", "two_kernel": "k_calculate_flux+k_update_cell",
+              "graph_two_kernel": "k_calculate_flux_tick+k_update_cell"}[a.mode]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{WORKLOAD}_f64_traffic.json")
     if os.path.exists(tpath):

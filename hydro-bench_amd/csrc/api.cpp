@@ -650,6 +650,56 @@ template <class R, int VAR> int steps_graph(hydro_mesh *m, int n, hipStream_t s)
     return HYDRO_OK;
 }
 
+// TWO_KERNEL steps captured into a hipGraph: `chunk` pairs of flux and
+// update launches on the reference layout, the flux kernels reading the
+// step's (day, kt) from the device clock, then the clock advance, so every
+// replay moves the boundary series on.  The layout's buffers do not
+// alternate, so one graph serves every replay.
+template <class R, int VAR> int build_graph_two_kernel(hydro_mesh *m, int chunk, GraphExec &ge) {
+    hipStream_t cs = m->cap_stream;
+    const ExtView<R> ev = ext_view<R>(m);
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < chunk && err == hipSuccess; k++) {
+        err = launch_calculate_flux_tick<R, VAR>(ev, m->tick.as<StepClock>(), k, m->spd, m->block_two, cs);
+        if (err == hipSuccess) err = launch_update_cell<R, VAR>(ev, m->block_two, cs);
+    }
+    if (err == hipSuccess) err = launch_clock_advance(m->tick.as<StepClock>(), chunk, m->spd, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &ge.g);
+    if (err != hipSuccess) return fail(HYDRO_ERR_HIP, "graph capture launch failed: %s", hipGetErrorString(err));
+    if (e2 != hipSuccess) return fail(HYDRO_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    HIP_TRY(hipGraphInstantiate(&ge.x, ge.g, nullptr, nullptr, 0));
+    return HYDRO_OK;
+}
+
+template <class R, int VAR> int steps_graph_two_kernel(hydro_mesh *m, int n, hipStream_t s) {
+    int rc = ensure_external<R>(m, s);
+    if (rc) return rc;
+    if (!m->tick.p && (rc = m->tick.alloc(sizeof(StepClock)))) return rc;
+    const int chunk = std::max(1, m->tune.graph_chunk);
+    if (!m->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking));
+    int done = 0;
+    if (n >= chunk) {
+        GraphKey key{HYDRO_MODE_GRAPH_TWO_KERNEL, 0, chunk};
+        auto it = m->graphs.find(key);
+        if (it == m->graphs.end()) {
+            auto ge = std::make_unique<GraphExec>();
+            if ((rc = build_graph_two_kernel<R, VAR>(m, chunk, *ge))) return rc;
+            it = m->graphs.emplace(key, std::move(ge)).first;
+        }
+        HIP_TRY(launch_clock_set(m->tick.as<StepClock>(), m->day, m->kt, s));
+        while (n - done >= chunk) {
+            HIP_TRY(hipGraphLaunch(it->second->x, s));
+            for (int k = 0; k < chunk; k++) advance(m);
+            done += chunk;
+            m->last_launches += 2 * chunk;
+        }
+        m->int_valid = false;
+    }
+    if (done < n) return steps_two_kernel<R, VAR>(m, n - done, s);
+    return HYDRO_OK;
+}
+
 template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream_t s) {
     int rc = check_persistent(m, s);
     if (rc || (rc = ensure_internal<R>(m, s))) return rc;
@@ -694,6 +744,7 @@ template <class R, int VAR> int step_dispatch(hydro_mesh *m, int n, int mode, hi
     case HYDRO_MODE_FUSED: return steps_fused<R, VAR>(m, n, s);
     case HYDRO_MODE_GRAPH: return steps_graph<R, VAR>(m, n, s);
     case HYDRO_MODE_PERSISTENT: return steps_persistent<R, VAR>(m, n, s);
+    case HYDRO_MODE_GRAPH_TWO_KERNEL: return steps_graph_two_kernel<R, VAR>(m, n, s);
     default: return fail(HYDRO_ERR_UNSUPPORTED, "mode %d not supported", mode);
     }
 }

@@ -483,6 +483,17 @@ k_fused_step_tick_o4(FusedView<R> f, const StepClock *clock, int offset, int spd
     tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, day);
 }
 
+// The reference layout's flux kernel with the step's time from the device
+// clock, for the graph-captured two-kernel mode.
+template <class R, int VAR>
+__global__ void __launch_bounds__(kMaxBlock) k_calculate_flux_tick(ExtView<R> m, const StepClock *clock, int offset,
+                                                                   int spd) {
+    int day, kt;
+    clock_at(*clock, offset, spd, &day, &kt);
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < 4 * m.CELL) ext_flux_slot<R, VAR>(m, idx, kt, day);
+}
+
 __global__ void k_tick_advance(int64_t *tick, int by) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *tick += by;
 }
@@ -718,6 +729,16 @@ hipError_t launch_calculate_flux(const ExtView<R> &m, int kt, int jt, int block,
 }
 
 template <class R, int VAR>
+hipError_t launch_calculate_flux_tick(const ExtView<R> &m, const StepClock *clock, int offset, int spd, int block,
+                                      hipStream_t s) {
+    const int n = 4 * m.CELL;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_calculate_flux_tick<R, VAR>), dim3(div_up(n, block)), dim3(block), 0, s, m, clock, offset,
+                       spd);
+    return hipGetLastError();
+}
+
+template <class R, int VAR>
 hipError_t launch_update_cell(const ExtView<R> &m, int block, hipStream_t s) {
     if (m.CELL == 0) return hipSuccess;
     hipLaunchKernelGGL((k_update_cell<R, VAR>), dim3(div_up(m.CELL, block)), dim3(block), 0, s, m);
@@ -887,6 +908,8 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
 
 #define HYDRO_INSTANTIATE(R, VAR)                                                                        \
     template hipError_t launch_calculate_flux<R, VAR>(const ExtView<R> &, int, int, int, hipStream_t);   \
+    template hipError_t launch_calculate_flux_tick<R, VAR>(const ExtView<R> &, const StepClock *, int, int, int,  \
+                                                           hipStream_t);                                         \
     template hipError_t launch_update_cell<R, VAR>(const ExtView<R> &, int, hipStream_t);               \
     template hipError_t launch_fused_step<R, VAR>(const FusedView<R> &, int, int, int, int, hipStream_t); \
     template hipError_t launch_fused_step_tick<R, VAR>(const FusedView<R> &, const StepClock *, int, int, \

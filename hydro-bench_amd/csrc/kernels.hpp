@@ -113,6 +113,9 @@ inline size_t tile_lds_bytes(int block, int rsize) { return (size_t)block * 18 *
 template <class R, int VAR>
 hipError_t launch_calculate_flux(const ExtView<R> &m, int kt, int jt, int block, hipStream_t s);
 template <class R, int VAR>
+hipError_t launch_calculate_flux_tick(const ExtView<R> &m, const StepClock *clock, int offset, int spd, int block,
+                                      hipStream_t s);
+template <class R, int VAR>
 hipError_t launch_update_cell(const ExtView<R> &m, int block, hipStream_t s);
 template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int xcd_remap, hipStream_t s);

@@ -167,7 +167,8 @@ def run(mesh: HostMesh, steps: Optional[int] = None, mode: str = "two_kernel", v
     The returned tensors are the live device arrays; in reordered modes they
     are refreshed by ``sync_fn()`` (and before every ``on_step``).
     """
-    dm = DeviceMesh(mesh, variant=variant, device=device, tune=tune, prepare=(mode != "two_kernel"))
+    dm = DeviceMesh(mesh, variant=variant, device=device, tune=tune,
+                    prepare=(mode not in ("two_kernel", "graph_two_kernel")))
     total = mesh.max_steps if steps is None else int(steps)
     if total > mesh.max_steps:
         raise _lib.HydroError(_lib.ERR_RANGE, f"{total} steps exceed the NDAYS cap {mesh.max_steps}")

@@ -74,8 +74,11 @@ typedef enum hydro_mode {
     HYDRO_MODE_TWO_KERNEL = 0, /* calculate_flux + update_cell launches on the reference layout */
     HYDRO_MODE_FUSED = 1,      /* one fused flux+update launch per step on the reordered layout */
     HYDRO_MODE_GRAPH = 2,      /* FUSED steps captured once into a hipGraph and replayed */
-    HYDRO_MODE_PERSISTENT = 3, /* one launch for all steps, grid-wide barrier between steps */
-    HYDRO_MODE_GRAPH_TWO_KERNEL = 4 /* TWO_KERNEL steps captured into a hipGraph */
+    HYDRO_MODE_PERSISTENT = 3, /* one launch for all steps; a tile starts a step once the tiles
+                                  it shares an edge with have finished the previous one (the
+                                  CELL scheme: a grid-wide barrier between steps) */
+    HYDRO_MODE_GRAPH_TWO_KERNEL = 4 /* TWO_KERNEL steps captured into a hipGraph; the flux kernel
+                                       reads (day, kt) from a device clock, so replays advance time */
 } hydro_mode;
 
 /* Mesh descriptor: the reference MeshView (hydro-cal/SOURCE/include/mesh/mesh.hpp:114-142)

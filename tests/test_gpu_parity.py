@@ -74,17 +74,17 @@ def test_channel_100_steps_fp64(oracle, variant):
 
 
 def test_modes_bitwise_identical():
-    """All four execution modes produce the same bits (fp64 and fp32)."""
+    """All five execution modes produce the same bits (fp64 and fp32)."""
     for dtype in (np.float64, np.float32):
         host = cached_mesh("channel_6675", dtype=dtype)
         res = {}
-        for mode in ("two_kernel", "fused", "graph", "persistent"):
+        for mode in ("two_kernel", "fused", "graph", "persistent", "graph_two_kernel"):
             dm = _device_mesh(host, graph_chunk=4)
             dm.step(37, mode)
             res[mode] = dm.fetch(*FIELDS)
             assert dm.get_time() == (0, 38, 60)
             dm.close()
-        for mode in ("fused", "graph", "persistent"):
+        for mode in ("fused", "graph", "persistent", "graph_two_kernel"):
             for k in FIELDS:
                 assert np.array_equal(res[mode][k].view(np.uint8), res["two_kernel"][k].view(np.uint8)), \
                     f"{np.dtype(dtype).name} {mode} {k} differs from two_kernel"
@@ -94,16 +94,36 @@ def test_mode_switching_mid_run(oracle):
     """Mixed modes on one mesh hand the state across layouts correctly."""
     host = cached_mesh("channel_6675")
     orc = oracle.OracleRun(host)
-    orc.step(40)
+    orc.step(50)
     dm = _device_mesh(host)
     dm.step(10, "fused")
     dm.step(10, "two_kernel")
     dm.step(10, "persistent")
     dm.step(10, "graph")
+    dm.step(10, "graph_two_kernel")
     got = dm.fetch(*FIELDS)
     for k in FIELDS:
         _check(got[k], orc[k], 1e-12, f"mixed {k}")
     dm.close()
+
+
+def test_graph_two_kernel_replays_advance_time():
+    """The graph-captured two-kernel mode reads each step's (day, kt) from a
+    device clock: 130 steps across two day boundaries (the inflow hydrograph
+    changes with time) in replays of 16, then 2 plain steps, give the bits of
+    130 plain two-kernel steps, fluxes included, in fp64 and fp32."""
+    for dtype in (np.float64, np.float32):
+        host = cached_mesh("channel_6675", dtype=dtype)
+        res = {}
+        for mode in ("two_kernel", "graph_two_kernel"):
+            dm = _device_mesh(host, graph_chunk=16)
+            dm.step(130, mode)
+            assert dm.get_time() == (2, 13, 60)
+            res[mode] = dm.fetch(*(FIELDS + FLUXES))
+            dm.close()
+        for k in FIELDS + FLUXES:
+            assert np.array_equal(res["graph_two_kernel"][k].view(np.uint8), res["two_kernel"][k].view(np.uint8)), \
+                f"{np.dtype(dtype).name} {k}"
 
 
 def test_floodplain_207k_step1_and_10(oracle):
