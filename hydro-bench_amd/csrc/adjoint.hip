@@ -40,6 +40,18 @@ __device__ __forceinline__ BcCtx<double> bc_of(const ExtView<double> &m, int cel
     return bc;
 }
 
+// Boundary context over an fp32 mesh's series, for the adjoint of an fp32
+// trajectory: slot_flux_bar reads QT, DQT, ZT, DZT and the slot's SIDE.
+struct BcF32 {
+    const float *QT, *DQT, *ZT, *DZT;
+    int CELL, cell, kt, jt;
+    double SIDE;
+};
+
+__device__ __forceinline__ BcF32 bc_of(const ExtView<float> &m, int cell, int slot, int kt, int jt) {
+    return BcF32{m.QT, m.DQT, m.ZT, m.DZT, m.CELL, cell, kt, jt, (double)m.SIDE[slot]};
+}
+
 __device__ __forceinline__ BcCtx<double> bc_none() {
     BcCtx<double> bc;
     bc.QT = bc.DQT = bc.ZT = bc.DZT = bc.ZW = bc.QW = bc.BF = nullptr;
@@ -158,6 +170,19 @@ __global__ void k_adj_flux(FusedView<double> f, const uint8_t *twin, const State
 // seed[i] is added to the Z adjoint in that order (as k_gauge_seed adds it).
 constexpr int kAdjCells = 32;
 
+// The step's scalars in double (an fp32 mesh's are its float values).
+template <class R> __device__ __forceinline__ Scalars<double> scalars64(const Scalars<R> &S) {
+    return Scalars<double>{(double)S.DT, (double)S.HM1, (double)S.HM2, (double)S.JL};
+}
+
+template <class R> __device__ __forceinline__ auto bc_empty_like(const ExtView<R> &) {
+    if constexpr (sizeof(R) == 8) {
+        return bc_none();
+    } else {
+        return BcF32{nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0, 0.0};
+    }
+}
+
 __device__ __forceinline__ double seeded_lz(const State<double> &l, int p, const int32_t *gmap, const int32_t *gnext,
                                             const double *seed) {
     double z = l.Z;
@@ -166,19 +191,24 @@ __device__ __forceinline__ double seeded_lz(const State<double> &l, int p, const
     return z;
 }
 
-__device__ __forceinline__ UpdateBar cell_bar(const FusedView<double> &f, const State<double> *s_in,
-                                              const double *wsum, const State<double> *lam_out, const int32_t *gmap,
+template <class R>
+__device__ __forceinline__ UpdateBar cell_bar(const FusedView<R> &f, const State<R> *s_in, const R *wsum,
+                                              const State<double> *lam_out, const int32_t *gmap,
                                               const int32_t *gnext, const double *seed, int q) {
     const int C = f.C;
-    const State<double> s = s_in[q];
+    const State<R> s = s_in[q];
     const State<double> l = lam_out[q];
     const double lz = seeded_lz(l, q, gmap, gnext, seed);
-    return update_bar(f.S, s.H, s.U, s.V, f.AREA[q], f.FNC[q], wsum[q], wsum[C + q], wsum[2 * C + q], l.H, l.U, l.V,
-                      lz);
+    return update_bar(scalars64(f.S), (double)s.H, (double)s.U, (double)s.V, (double)f.AREA[q], (double)f.FNC[q],
+                      (double)wsum[q], (double)wsum[C + q], (double)wsum[2 * C + q], l.H, l.U, l.V, lz);
 }
 
-__global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<double> f, const uint8_t *twin,
-                                                            const State<double> *s_in, const double *wsum,
+// R is the trajectory's type: the recorded states, flux sums and the mesh
+// statics are read in R and promoted; the adjoint arithmetic, the adjoint
+// states and the gradients are double.
+template <class R>
+__global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<R> f, const uint8_t *twin,
+                                                            const State<R> *s_in, const R *wsum,
                                                             const State<double> *lam_out, State<double> *lam_in,
                                                             double *g_fnc, double *g_zbc, const int32_t *gmap,
                                                             const int32_t *gnext, const double *seed, int kt, int jt,
@@ -214,12 +244,12 @@ __global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<double> f,
     SideBar acc;
     bool bad = false;
     if (kind != K_PAD && (!is_twin || twin_on)) {
-        const State<double> sp = s_in[p];
-        const SideIn me{sp.H, sp.U, sp.V, sp.Z, f.ZBC[p]};
+        const State<R> sp = s_in[p];
+        const SideIn me{sp.H, sp.U, sp.V, sp.Z, (double)f.ZBC[p]};
         SideIn nb{0, 0, 0, 0, 0};
         if (q >= 0) {
-            const State<double> sq = s_in[q];
-            nb = SideIn{sq.H, sq.U, sq.V, sq.Z, f.ZBC[q]};
+            const State<R> sq = s_in[q];
+            nb = SideIn{sq.H, sq.U, sq.V, sq.Z, (double)f.ZBC[q]};
         }
         // the slot whose flux is reversed: this cell's slot j, or the
         // neighbour's slot j2 with the roles of the two states swapped
@@ -233,18 +263,23 @@ __global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<double> f,
             ws.WU = wb[1][l];
             ws.WV = wb[2][l];
         }
-        const double c = f.COSF[es], sn = f.SINF[es], SL = f.SIDE[es];
-        const double SLCA = f.SLCOS ? f.SLCOS[es] : SL * c;
-        const double SLSA = f.SLSIN ? f.SLSIN[es] : SL * sn;
+        // SIDE*COSF and SIDE*SINF in the trajectory's type, as the forward
+        // step forms them
+        const R cR = f.COSF[es], snR = f.SINF[es], SLR = f.SIDE[es];
+        const double c = cR, sn = snR, SL = SLR;
+        const double SLCA = f.SLCOS ? (double)f.SLCOS[es] : (double)(R)(SLR * cR);
+        const double SLSA = f.SLSIN ? (double)f.SLSIN[es] : (double)(R)(SLR * snR);
         double Fb[4];
         flux_bar_of_sums(ws, SL, SLCA, SLSA, Fb);
         const int oc = is_twin ? q : p, js = is_twin ? j2 : j;
-        const BcCtx<double> bc = (ks >= K_B10) ? bc_of(*f.ext, f.orig[oc], 4 * f.orig[oc] + js, kt, jt) : bc_none();
+        auto bc = bc_empty_like(*f.ext);
+        if (ks >= K_B10) bc = bc_of(*f.ext, f.orig[oc], 4 * f.orig[oc] + js, kt, jt);
+        const Scalars<double> S = scalars64(f.S);
         SideBar other;
         if (is_twin)
-            bad |= !slot_flux_bar(ks, f.S, bc, nb, true, me, c, sn, Fb, other, acc);
+            bad |= !slot_flux_bar(ks, S, bc, nb, true, me, c, sn, Fb, other, acc);
         else
-            bad |= !slot_flux_bar(ks, f.S, bc, me, q >= 0, nb, c, sn, Fb, acc, other);
+            bad |= !slot_flux_bar(ks, S, bc, me, q >= 0, nb, c, sn, Fb, acc, other);
     }
     const int k = is_twin ? 1 : 0;
     part[j][k][0][l] = acc.H;
@@ -361,7 +396,8 @@ __global__ void k_tlm_step(FusedView<double> f, const State<double> *s_in, const
 }
 
 // gauges: value of Z (or add a seed into lam.Z) at ORIGINAL cell ids
-__global__ void k_gauge_gather(const State<double> *st, const int32_t *inv, const int32_t *cells, int n, double *out) {
+template <class R>
+__global__ void k_gauge_gather(const State<R> *st, const int32_t *inv, const int32_t *cells, int n, double *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = st[inv[cells[i]]].Z;
 }
@@ -418,15 +454,24 @@ hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, 
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint_step_fused(const FusedView<double> &f, const uint8_t *twin, const State<double> *s_in,
-                                     const double *wsum, const State<double> *lam_out, State<double> *lam_in,
-                                     double *g_fnc, double *g_zbc, const int32_t *gmap, const int32_t *gnext,
-                                     const double *seed, int kt, int jt, int *unsupported, hipStream_t s) {
+template <class R>
+hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const State<R> *s_in, const R *wsum,
+                                     const State<double> *lam_out, State<double> *lam_in, double *g_fnc,
+                                     double *g_zbc, const int32_t *gmap, const int32_t *gnext, const double *seed,
+                                     int kt, int jt, int *unsupported, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_step, dim3(div_up(f.C, kAdjCells)), dim3(8 * kAdjCells), 0, s, f, twin, s_in, wsum,
-                       lam_out, lam_in, g_fnc, g_zbc, gmap, gnext, seed, kt, jt, unsupported);
+    hipLaunchKernelGGL((k_adj_step<R>), dim3(div_up(f.C, kAdjCells)), dim3(8 * kAdjCells), 0, s, f, twin, s_in,
+                       wsum, lam_out, lam_in, g_fnc, g_zbc, gmap, gnext, seed, kt, jt, unsupported);
     return hipGetLastError();
 }
+template hipError_t launch_adjoint_step_fused<double>(const FusedView<double> &, const uint8_t *,
+                                                      const State<double> *, const double *, const State<double> *,
+                                                      State<double> *, double *, double *, const int32_t *,
+                                                      const int32_t *, const double *, int, int, int *, hipStream_t);
+template hipError_t launch_adjoint_step_fused<float>(const FusedView<float> &, const uint8_t *, const State<float> *,
+                                                     const float *, const State<double> *, State<double> *, double *,
+                                                     double *, const int32_t *, const int32_t *, const double *, int,
+                                                     int, int *, hipStream_t);
 
 hipError_t launch_gauge_map(const int32_t *inv, const int32_t *cells, int n, int C, int32_t *gmap, int32_t *gnext,
                             int32_t *internal, hipStream_t s) {
@@ -444,12 +489,17 @@ hipError_t launch_tlm_step(const FusedView<double> &f, const State<double> *s_in
     return hipGetLastError();
 }
 
-hipError_t launch_gauge_gather(const State<double> *st, const int32_t *inv, const int32_t *cells, int n, double *out,
+template <class R>
+hipError_t launch_gauge_gather(const State<R> *st, const int32_t *inv, const int32_t *cells, int n, double *out,
                                hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gauge_gather, dim3(div_up(n, 256)), dim3(256), 0, s, st, inv, cells, n, out);
+    hipLaunchKernelGGL((k_gauge_gather<R>), dim3(div_up(n, 256)), dim3(256), 0, s, st, inv, cells, n, out);
     return hipGetLastError();
 }
+template hipError_t launch_gauge_gather<double>(const State<double> *, const int32_t *, const int32_t *, int, double *,
+                                                hipStream_t);
+template hipError_t launch_gauge_gather<float>(const State<float> *, const int32_t *, const int32_t *, int, double *,
+                                               hipStream_t);
 
 hipError_t launch_gauge_seed(State<double> *lam, const int32_t *inv, const int32_t *cells, int n, const double *seed,
                              hipStream_t s) {

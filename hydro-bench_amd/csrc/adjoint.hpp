@@ -157,8 +157,11 @@ __host__ __device__ inline double urb_tan(double QLn, double H_pre, double HB1, 
 
 // ---------------------------------------------------------------------------
 // Reverse of slot_flux<double, 0> for one slot.  Fb = adjoint of (F0..F3).
-// Returns false for slot kinds without an adjoint.
-__host__ __device__ inline bool slot_flux_bar(int kind, const Scalars<double> &S, const BcCtx<double> &bc,
+// Returns false for slot kinds without an adjoint.  BcT is BcCtx<double>, or
+// for an fp32 trajectory a context whose boundary series are the fp32 arrays
+// (their values enter the recomputation as they are stored).
+template <class BcT>
+__host__ __device__ inline bool slot_flux_bar(int kind, const Scalars<double> &S, const BcT &bc,
                                               const SideIn &me, bool has, const SideIn &nb, double COSJ,
                                               double SINJ, const double Fb[4], SideBar &meb, SideBar &nbb) {
     using K = Consts<double>;

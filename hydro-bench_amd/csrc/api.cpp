@@ -1138,10 +1138,13 @@ int hydro_last_timing(const hydro_mesh *m, double *total_ms, int *launches, int 
 
 // ---- adjoint / tangent-linear support ---------------------------------------
 
-static int adj_ready(hydro_mesh *m) {
+// any_dtype: the misfit loops and the maps also take fp32 meshes (an fp32
+// trajectory with a double adjoint); the single-step API is fp64 only.
+static int adj_ready(hydro_mesh *m, bool any_dtype = false) {
     if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
-    if (m->d.dtype != HYDRO_F64 || m->d.variant != HYDRO_VARIANT_A)
-        return fail(HYDRO_ERR_UNSUPPORTED, "adjoint and tangent-linear steps need an fp64 variant-A mesh");
+    if ((m->d.dtype != HYDRO_F64 && !any_dtype) || m->d.variant != HYDRO_VARIANT_A)
+        return fail(HYDRO_ERR_UNSUPPORTED, any_dtype ? "the misfit loops need a variant-A mesh"
+                                                     : "adjoint and tangent-linear steps need an fp64 variant-A mesh");
     if (!m->prepared) {
         const hydro_tune t = m->tune;
         int rc = hydro_prepare(m, &t);
@@ -1259,9 +1262,12 @@ static int check_gauges(hydro_mesh *m, const int32_t *cells, int n) {
 }
 
 int hydro_gauge_values(hydro_mesh *m, const void *state, const int32_t *cells, int n, double *out, hipStream_t s) {
-    int rc = adj_ready(m);
+    int rc = adj_ready(m, true);
     if (rc || (rc = check_gauges(m, cells, n))) return rc;
-    HIP_TRY(launch_gauge_gather((const State<double> *)state, m->inv.as<int32_t>(), cells, n, out, s));
+    if (m->d.dtype == HYDRO_F64)
+        HIP_TRY(launch_gauge_gather((const State<double> *)state, m->inv.as<int32_t>(), cells, n, out, s));
+    else
+        HIP_TRY(launch_gauge_gather((const State<float> *)state, m->inv.as<int32_t>(), cells, n, out, s));
     return HYDRO_OK;
 }
 
@@ -1285,41 +1291,85 @@ static int gauge_tables(hydro_mesh *m, const int32_t *gauges, int n, hipStream_t
     return HYDRO_OK;
 }
 
-int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const int32_t *gauges, int n, double *zg,
-                         hipStream_t s) {
-    int rc = adj_ready(m);
-    if (rc || (rc = check_gauges(m, gauges, n))) return rc;
-    if (nsteps < 0 || seg <= 0 || (n > 0 && !zg)) return fail(HYDRO_ERR_INVALID, "bad argument");
-    const int per = m->spd - 1;
-    const int64_t remaining = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
-    if (nsteps > remaining) return fail(HYDRO_ERR_RANGE, "%d steps requested, %lld remain", nsteps, (long long)remaining);
-    if ((rc = ensure_internal<double>(m, s)) || (rc = gauge_tables(m, gauges, n, s))) return rc;
+extern "C++" template <class R>
+static int misfit_forward_t(hydro_mesh *m, int nsteps, int seg, void *ckpt, const int32_t *gauges, int n, double *zg,
+                            hipStream_t s) {
+    int rc;
+    if ((rc = ensure_internal<R>(m, s)) || (rc = gauge_tables(m, gauges, n, s))) return rc;
     const size_t C = (size_t)m->CELL;
     for (int k = 0; k < nsteps; k++) {
         if (ckpt && k % seg == 0)
-            HIP_TRY(hipMemcpyAsync((State<double> *)ckpt + (size_t)(k / seg) * C, m->st[m->cur].p,
-                                   m->st[m->cur].bytes, hipMemcpyDeviceToDevice, s));
-        FusedView<double> f = fused_view<double>(m, m->cur);
+            HIP_TRY(hipMemcpyAsync((State<R> *)ckpt + (size_t)(k / seg) * C, m->st[m->cur].p, m->st[m->cur].bytes,
+                                   hipMemcpyDeviceToDevice, s));
+        FusedView<R> f = fused_view<R>(m, m->cur);
         if (n > 0 && k > 0) {   // this step's input is the previous step's output
             f.gauge_cells = m->gcells.as<int32_t>();
             f.gauge_n = n;
             f.gauge_out = zg + (size_t)(k - 1) * n;
         }
-        HIP_TRY((launch_fused_step<double, 0>(f, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
+        HIP_TRY((launch_fused_step<R, 0>(f, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
         m->cur ^= 1;
         advance(m);
     }
     if (n > 0 && nsteps > 0)
-        HIP_TRY(launch_gauge_gather(m->st[m->cur].as<State<double>>(), m->inv.as<int32_t>(), gauges, n,
+        HIP_TRY(launch_gauge_gather(m->st[m->cur].as<State<R>>(), m->inv.as<int32_t>(), gauges, n,
                                     zg + (size_t)(nsteps - 1) * n, s));
     m->ext_valid = false;
+    return HYDRO_OK;
+}
+
+int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const int32_t *gauges, int n, double *zg,
+                         hipStream_t s) {
+    int rc = adj_ready(m, true);
+    if (rc || (rc = check_gauges(m, gauges, n))) return rc;
+    if (nsteps < 0 || seg <= 0 || (n > 0 && !zg)) return fail(HYDRO_ERR_INVALID, "bad argument");
+    const int per = m->spd - 1;
+    const int64_t remaining = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
+    if (nsteps > remaining) return fail(HYDRO_ERR_RANGE, "%d steps requested, %lld remain", nsteps, (long long)remaining);
+    return m->d.dtype == HYDRO_F64 ? misfit_forward_t<double>(m, nsteps, seg, ckpt, gauges, n, zg, s)
+                                   : misfit_forward_t<float>(m, nsteps, seg, ckpt, gauges, n, zg, s);
+}
+
+extern "C++" template <class R>
+static int misfit_backward_t(hydro_mesh *m, int nsteps, int seg, const void *ckpt, int n, const double *resid,
+                             void *states, void *wsums, void *lam, double *g_fnc, double *g_zbc, int *lam_index,
+                             hipStream_t s) {
+    const int per = m->spd - 1;
+    const int64_t g0 = global_step(m);
+    const size_t C = (size_t)m->CELL;
+    State<double> *L[2] = {(State<double> *)lam, (State<double> *)lam + C};
+    HIP_TRY(hipMemsetAsync(L[0], 0, C * sizeof(State<double>), s));
+    int rc, cur = 0;
+    const int nseg = (nsteps + seg - 1) / seg;
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int s0 = k * seg;
+        const int len = std::min(seg, nsteps - s0);
+        HIP_TRY(hipMemcpyAsync(m->st[m->cur].p, (const State<R> *)ckpt + (size_t)k * C, m->st[m->cur].bytes,
+                               hipMemcpyDeviceToDevice, s));
+        m->int_valid = true;
+        m->ext_valid = false;
+        m->day = (int)((g0 + s0) / per);
+        m->kt = 1 + (int)((g0 + s0) % per);
+        if ((rc = record_t<R, 0>(m, len, states, wsums, s))) return rc;
+        const FusedView<R> f = fused_view<R>(m, m->cur);
+        for (int i = len - 1; i >= 0; i--) {
+            const int64_t g = g0 + s0 + i;
+            HIP_TRY(launch_adjoint_step_fused<R>(f, m->twin.as<uint8_t>(), (const State<R> *)states + (size_t)i * C,
+                                                 (const R *)wsums + (size_t)i * 3 * C, L[cur], L[cur ^ 1], g_fnc,
+                                                 g_zbc, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
+                                                 n > 0 ? resid + (size_t)(s0 + i) * n : nullptr, 1 + (int)(g % per),
+                                                 (int)(g / per), m->adj_flag.as<int>(), s));
+            cur ^= 1;
+        }
+    }
+    *lam_index = cur;
     return HYDRO_OK;
 }
 
 int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, const int32_t *gauges, int n,
                           const double *resid, void *states, void *wsums, void *lam, double *g_fnc, double *g_zbc,
                           int *lam_index, hipStream_t s) {
-    int rc = adj_ready(m);
+    int rc = adj_ready(m, true);
     if (rc || (rc = check_gauges(m, gauges, n))) return rc;
     if (nsteps < 0 || seg <= 0 || !ckpt || (n > 0 && !resid) || !states || !wsums || !lam || !g_fnc || !g_zbc ||
         !lam_index)
@@ -1334,38 +1384,15 @@ int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, 
         if ((rc = m->adj_flag.alloc(sizeof(int)))) return rc;
         HIP_TRY(hipMemset(m->adj_flag.p, 0, sizeof(int)));
     }
-    const size_t C = (size_t)m->CELL;
-    State<double> *L[2] = {(State<double> *)lam, (State<double> *)lam + C};
-    HIP_TRY(hipMemsetAsync(L[0], 0, C * sizeof(State<double>), s));
-    int cur = 0;
-    const int nseg = (nsteps + seg - 1) / seg;
-    for (int k = nseg - 1; k >= 0; k--) {
-        const int s0 = k * seg;
-        const int len = std::min(seg, nsteps - s0);
-        HIP_TRY(hipMemcpyAsync(m->st[m->cur].p, (const State<double> *)ckpt + (size_t)k * C, m->st[m->cur].bytes,
-                               hipMemcpyDeviceToDevice, s));
-        m->int_valid = true;
-        m->ext_valid = false;
-        m->day = (int)((g0 + s0) / per);
-        m->kt = 1 + (int)((g0 + s0) % per);
-        if ((rc = record_t<double, 0>(m, len, states, wsums, s))) return rc;
-        const FusedView<double> f = fused_view<double>(m, m->cur);
-        for (int i = len - 1; i >= 0; i--) {
-            const int64_t g = g0 + s0 + i;
-            HIP_TRY(launch_adjoint_step_fused(f, m->twin.as<uint8_t>(), (const State<double> *)states + (size_t)i * C,
-                                              (const double *)wsums + (size_t)i * 3 * C, L[cur], L[cur ^ 1], g_fnc,
-                                              g_zbc, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
-                                              n > 0 ? resid + (size_t)(s0 + i) * n : nullptr, 1 + (int)(g % per),
-                                              (int)(g / per), m->adj_flag.as<int>(), s));
-            cur ^= 1;
-        }
-    }
-    *lam_index = cur;
-    return HYDRO_OK;
+    return m->d.dtype == HYDRO_F64
+               ? misfit_backward_t<double>(m, nsteps, seg, ckpt, n, resid, states, wsums, lam, g_fnc, g_zbc,
+                                           lam_index, s)
+               : misfit_backward_t<float>(m, nsteps, seg, ckpt, n, resid, states, wsums, lam, g_fnc, g_zbc,
+                                          lam_index, s);
 }
 
 int hydro_cells_map(hydro_mesh *m, const double *in, double *out, int to_external, hipStream_t s) {
-    int rc = adj_ready(m);
+    int rc = adj_ready(m, true);
     if (rc) return rc;
     HIP_TRY(launch_cells_map(in, m->orig.as<int32_t>(), out, m->CELL, to_external, s));
     return HYDRO_OK;
@@ -1373,7 +1400,7 @@ int hydro_cells_map(hydro_mesh *m, const double *in, double *out, int to_externa
 
 int hydro_states_map(hydro_mesh *m, void *state, double *H, double *U, double *V, double *Z, int to_external,
                      hipStream_t s) {
-    int rc = adj_ready(m);
+    int rc = adj_ready(m, true);
     if (rc) return rc;
     HIP_TRY(launch_states_map((State<double> *)state, m->orig.as<int32_t>(), H, U, V, Z, m->CELL, to_external, s));
     return HYDRO_OK;

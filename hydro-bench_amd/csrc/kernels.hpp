@@ -186,13 +186,15 @@ hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, 
 hipError_t launch_tlm_step(const FusedView<double> &f, const State<double> *s_in, const State<double> *ds_in,
                            const double *dfnc, const double *dzbc, State<double> *s_out, State<double> *ds_out, int kt,
                            int jt, hipStream_t s);
-hipError_t launch_adjoint_step_fused(const FusedView<double> &f, const uint8_t *twin, const State<double> *s_in,
-                                     const double *wsum, const State<double> *lam_out, State<double> *lam_in,
-                                     double *g_fnc, double *g_zbc, const int32_t *gmap, const int32_t *gnext,
-                                     const double *seed, int kt, int jt, int *unsupported, hipStream_t s);
+template <class R>
+hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const State<R> *s_in, const R *wsum,
+                                     const State<double> *lam_out, State<double> *lam_in, double *g_fnc,
+                                     double *g_zbc, const int32_t *gmap, const int32_t *gnext, const double *seed,
+                                     int kt, int jt, int *unsupported, hipStream_t s);
 hipError_t launch_gauge_map(const int32_t *inv, const int32_t *cells, int n, int C, int32_t *gmap, int32_t *gnext,
                             int32_t *internal, hipStream_t s);
-hipError_t launch_gauge_gather(const State<double> *st, const int32_t *inv, const int32_t *cells, int n, double *out,
+template <class R>
+hipError_t launch_gauge_gather(const State<R> *st, const int32_t *inv, const int32_t *cells, int n, double *out,
                                hipStream_t s);
 hipError_t launch_gauge_seed(State<double> *lam, const int32_t *inv, const int32_t *cells, int n, const double *seed,
                              hipStream_t s);

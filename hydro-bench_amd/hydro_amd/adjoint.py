@@ -6,7 +6,11 @@ bed elevation, through the library's hand-written adjoint kernels.
     loss = GaugeMisfit.apply(n, zbc, problem)
 
 with ``n`` and ``zbc`` float64 tensors on the device in the caller's cell
-order.  The loss is
+order.  The mesh may be fp64 or fp32: an fp32 mesh runs the forward steps,
+the checkpoints and the recomputation in fp32, and the adjoint steps in
+double on the recorded fp32 states (the gradient of the step's arithmetic
+along the fp32 trajectory), accumulating the gradients in double.  The loss
+is
 
     L = sum_{steps s} sum_{gauges g} (Z_g^s - Zobs_g^s)^2
 
@@ -77,14 +81,18 @@ class _Buffers:
         C = dm.host.CELL
         dev = dm.device
         self.C = C
-        self.ckpt = torch.empty((nseg + 1, C, 4), dtype=torch.float64, device=dev)
-        self.states = torch.empty((seg + 1, C, 4), dtype=torch.float64, device=dev)
-        self.wsums = torch.empty((seg, 3, C), dtype=torch.float64, device=dev)
+        # the trajectory (checkpoints, recorded states and flux sums) in the
+        # mesh's type; adjoint states and gradients in double
+        tt = torch.float64 if dm.host.dtype == np.float64 else torch.float32
+        self.ckpt = torch.empty((nseg + 1, C, 4), dtype=tt, device=dev)
+        self.states = torch.empty((seg + 1, C, 4), dtype=tt, device=dev)
+        self.wsums = torch.empty((seg, 3, C), dtype=tt, device=dev)
         self.lam = torch.zeros((2, C, 4), dtype=torch.float64, device=dev)
         self.g_fnc = torch.zeros(C, dtype=torch.float64, device=dev)
         self.g_zbc = torch.zeros(C, dtype=torch.float64, device=dev)
-        self.scratch = torch.empty((C, 4), dtype=torch.float64, device=dev)
-        assert sb.value == C * 32 and wb.value == C * 24
+        self.scratch = torch.empty((C, 4), dtype=tt, device=dev)
+        rs = self.ckpt.element_size()
+        assert sb.value == C * 4 * rs and wb.value == C * 3 * rs
 
 
 def _set_params(dm: DeviceMesh, n, zbc):
@@ -92,8 +100,9 @@ def _set_params(dm: DeviceMesh, n, zbc):
     torch = _torch()
     with torch.no_grad():
         dm.t["FNC"].copy_(9.81 * n * n)
-        dm.t["ZBC"].copy_(zbc)
-        dm.t["ZB1"].copy_(zbc + dm.host.HM1)
+        zt = zbc.to(dm.t["ZBC"].dtype)   # ZB1 = ZBC + HM1 in the mesh's type, as the steps form it
+        dm.t["ZBC"].copy_(zt)
+        dm.t["ZB1"].copy_(zt + torch.tensor(dm.host.HM1, dtype=zt.dtype, device=zt.device))
     _lib.check(dm.lib.hydro_refresh_statics(dm.h, dm.stream()))
 
 
