@@ -241,7 +241,7 @@ int hydro_batch_destroy(hydro_batch *b);
 int hydro_batch_step_n(hydro_batch *b, int nsteps, int mode, hipStream_t stream);
 int hydro_batch_sync(hydro_batch *b, hipStream_t stream);
 
-/* ---- adjoint and tangent-linear steps (fp64, variant A) ------------------- */
+/* ---- adjoint and tangent-linear steps (variant A) -------------------------- */
 /* No counterpart in the reference: these make gradients of a gauge misfit
  * with respect to per-cell Manning n (through FNC = 9.81 n^2) and bed ZBC
  * available.  They work on the reordered layout; every buffer is caller-owned
@@ -251,7 +251,10 @@ int hydro_batch_sync(hydro_batch *b, hipStream_t stream);
  *   wsum    one step's flux sums [3][C] (wsum_bytes)
  *   cellvec one value per cell          (cellvec_bytes)
  * A step's time (kt, jt) is passed explicitly (the step that produced the
- * state after it). */
+ * state after it).  The single-step calls need an fp64 mesh; the misfit
+ * loops, the gauge reads and the maps also take an fp32 mesh, whose state and
+ * wsum buffers are then fp32 (the trajectory) while adjoint buffers, gauge
+ * values, residuals and gradients stay double. */
 int hydro_adj_sizes(const hydro_mesh *m, size_t *state_bytes, size_t *wsum_bytes, size_t *cellvec_bytes);
 /* current state <-> a state buffer (checkpoints) */
 int hydro_state_save(hydro_mesh *m, void *state, hipStream_t stream);
