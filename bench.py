@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--calib-precision", default="fp32", choices=["fp32", "fp64"],
                     help="trajectory precision of --calibrate (the adjoint and the gradients are fp64 either way)")
+    ap.add_argument("--calib-sequential", action="store_true",
+                    help="--calibrate one hypothesis after another instead of all in one scenario batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 512 MiB device-to-device 5 times (a kernel of known traffic, to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters by)")
@@ -301,7 +303,8 @@ def adjoint_bench(a):
 
 def calibrate_bench(a):
     """Manning calibration of the 207K floodplain: `hypotheses` starting
-    fields, `iters` gradient steps each, on an fp32 (default) or fp64
+    fields, `iters` gradient steps each, all hypotheses in one scenario batch
+    (default) or one after another, on an fp32 (default) or fp64
     trajectory; observations from the fp64 run of a truth field; every
     misfit reported is the fp64 forward run's (equal to the CPU oracle's)."""
     import numpy as np
@@ -309,7 +312,7 @@ def calibrate_bench(a):
 
     import hydro_amd
     from hydro_amd.adjoint import MisfitProblem, run_forward
-    from hydro_amd.calibrate import calibrate, smooth_field
+    from hydro_amd.calibrate import calibrate, calibrate_many, smooth_field
     from hydro_amd.device import DeviceMesh
 
     host = hydro_amd.make_config(WORKLOAD, dtype=np.float64)
@@ -345,13 +348,16 @@ def calibrate_bench(a):
 
     truth = field(5005)
     res = []
-    cal_wall = 0.0
-    for h in range(a.hypotheses):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        cal = calibrate(prob, t(field(6000 + h)), zbc, iters=a.iters)
-        torch.cuda.synchronize()
-        cal_wall += time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if a.calib_sequential:
+        cals = [calibrate(prob, t(field(6000 + h)), zbc, iters=a.iters) for h in range(a.hypotheses)]
+    else:
+        n0s = torch.stack([t(field(6000 + h)) for h in range(a.hypotheses)]).contiguous()
+        cals = calibrate_many(prob, n0s, zbc, iters=a.iters)
+    torch.cuda.synchronize()
+    cal_wall = time.perf_counter() - t0
+    for h, cal in enumerate(cals):
         n_final = cal.n.cpu().numpy()
         first64, final64 = misfit64(field(6000 + h)), misfit64(n_final)
         res.append({"hypothesis": h, "misfit_first": first64, "misfit_final": final64,
@@ -370,7 +376,8 @@ def calibrate_bench(a):
         "data": "synthetic (seeded 207,234-cell floodplain; truth and hypotheses smooth Manning fields, "
                 "mean 0.04, std 0.01; 64 gauges)",
         "config": {"workload": WORKLOAD, "cells": host.CELL, "steps": steps, "checkpoint_every": prob.seg,
-                   "hypotheses": a.hypotheses, "iterations": a.iters, "trajectory": a.calib_precision},
+                   "hypotheses": a.hypotheses, "iterations": a.iters, "trajectory": a.calib_precision,
+                   "batched": not a.calib_sequential},
         "wall_s": cal_wall,
         "ms_per_gradient": cal_wall * 1e3 / grads,
         "hypotheses": res,

@@ -192,27 +192,41 @@ __device__ __forceinline__ double seeded_lz(const State<double> &l, int p, const
 }
 
 template <class R>
-__device__ __forceinline__ UpdateBar cell_bar(const FusedView<R> &f, const State<R> *s_in, const R *wsum,
-                                              const State<double> *lam_out, const int32_t *gmap,
+__device__ __forceinline__ UpdateBar cell_bar(const FusedView<R> &f, const R *fnc, const State<R> *s_in,
+                                              const R *wsum, const State<double> *lam_out, const int32_t *gmap,
                                               const int32_t *gnext, const double *seed, int q) {
     const int C = f.C;
     const State<R> s = s_in[q];
     const State<double> l = lam_out[q];
     const double lz = seeded_lz(l, q, gmap, gnext, seed);
-    return update_bar(scalars64(f.S), (double)s.H, (double)s.U, (double)s.V, (double)f.AREA[q], (double)f.FNC[q],
+    return update_bar(scalars64(f.S), (double)s.H, (double)s.U, (double)s.V, (double)f.AREA[q], (double)fnc[q],
                       (double)wsum[q], (double)wsum[C + q], (double)wsum[2 * C + q], l.H, l.U, l.V, lz);
 }
 
 // R is the trajectory's type: the recorded states, flux sums and the mesh
 // statics are read in R and promoted; the adjoint arithmetic, the adjoint
-// states and the gradients are double.
+// states and the gradients are double.  blockIdx.y is the scenario of a
+// scenario batch (one mesh, S Manning fields and states): its state, flux
+// sums, adjoints, gradients and FNC sit blockIdx.y blocks of C (3C for the
+// sums) into the arrays, its gauge seeds nseed into `seed`.
 template <class R>
 __global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<R> f, const uint8_t *twin,
                                                             const State<R> *s_in, const R *wsum,
                                                             const State<double> *lam_out, State<double> *lam_in,
                                                             double *g_fnc, double *g_zbc, const int32_t *gmap,
-                                                            const int32_t *gnext, const double *seed, int kt, int jt,
-                                                            int *unsupported) {
+                                                            const int32_t *gnext, const double *seed, int nseed,
+                                                            int kt, int jt, int *unsupported) {
+    {
+        const size_t so = (size_t)blockIdx.y * f.C;
+        s_in += so;
+        wsum += 3 * so;
+        lam_out += so;
+        lam_in += so;
+        g_fnc += so;
+        g_zbc += so;
+        if (seed) seed += (size_t)blockIdx.y * nseed;
+    }
+    const R *fnc = f.FNC + (size_t)blockIdx.y * f.C;
     __shared__ double wb[3][kAdjCells];
     __shared__ double part[4][2][5][kAdjCells];
     const int C = f.C;
@@ -224,7 +238,7 @@ __global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<R> f, cons
     const bool valid = p < C;
     UpdateBar up;
     if (w == 0 && valid) {
-        up = cell_bar(f, s_in, wsum, lam_out, gmap, gnext, seed, p);
+        up = cell_bar(f, fnc, s_in, wsum, lam_out, gmap, gnext, seed, p);
         wb[0][l] = up.WH;
         wb[1][l] = up.WU;
         wb[2][l] = up.WV;
@@ -239,7 +253,7 @@ __global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<R> f, cons
     }
     const bool twin_on = kind2 != K_PAD;
     UpdateBar uq;
-    if (twin_on) uq = cell_bar(f, s_in, wsum, lam_out, gmap, gnext, seed, q);
+    if (twin_on) uq = cell_bar(f, fnc, s_in, wsum, lam_out, gmap, gnext, seed, q);
     __syncthreads();
     SideBar acc;
     bool bad = false;
@@ -396,6 +410,21 @@ __global__ void k_tlm_step(FusedView<double> f, const State<double> *s_in, const
 }
 
 // gauges: value of Z (or add a seed into lam.Z) at ORIGINAL cell ids
+// Z of the gauges of S scenario states (st [S][C]) to out [S][n].
+template <class R>
+__global__ void k_gauge_gather_scen(const State<R> *st, int C, const int32_t *inv, const int32_t *cells, int n,
+                                    double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[(size_t)blockIdx.y * n + i] = st[(size_t)blockIdx.y * C + inv[cells[i]]].Z;
+}
+
+// External-order double values [S][C] -> internal-order R values [S][C].
+template <class R>
+__global__ void k_scen_to_internal(const double *in, const int32_t *orig, R *out, int C) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < C) out[(size_t)blockIdx.y * C + p] = (R)in[(size_t)blockIdx.y * C + orig[p]];
+}
+
 template <class R>
 __global__ void k_gauge_gather(const State<R> *st, const int32_t *inv, const int32_t *cells, int n, double *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -458,20 +487,21 @@ template <class R>
 hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const State<R> *s_in, const R *wsum,
                                      const State<double> *lam_out, State<double> *lam_in, double *g_fnc,
                                      double *g_zbc, const int32_t *gmap, const int32_t *gnext, const double *seed,
-                                     int kt, int jt, int *unsupported, hipStream_t s) {
-    if (f.C == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_adj_step<R>), dim3(div_up(f.C, kAdjCells)), dim3(8 * kAdjCells), 0, s, f, twin, s_in,
-                       wsum, lam_out, lam_in, g_fnc, g_zbc, gmap, gnext, seed, kt, jt, unsupported);
+                                     int nseed, int nscen, int kt, int jt, int *unsupported, hipStream_t s) {
+    if (f.C == 0 || nscen <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_adj_step<R>), dim3(div_up(f.C, kAdjCells), nscen), dim3(8 * kAdjCells), 0, s, f, twin,
+                       s_in, wsum, lam_out, lam_in, g_fnc, g_zbc, gmap, gnext, seed, nseed, kt, jt, unsupported);
     return hipGetLastError();
 }
 template hipError_t launch_adjoint_step_fused<double>(const FusedView<double> &, const uint8_t *,
                                                       const State<double> *, const double *, const State<double> *,
                                                       State<double> *, double *, double *, const int32_t *,
-                                                      const int32_t *, const double *, int, int, int *, hipStream_t);
+                                                      const int32_t *, const double *, int, int, int, int, int *,
+                                                      hipStream_t);
 template hipError_t launch_adjoint_step_fused<float>(const FusedView<float> &, const uint8_t *, const State<float> *,
                                                      const float *, const State<double> *, State<double> *, double *,
                                                      double *, const int32_t *, const int32_t *, const double *, int,
-                                                     int, int *, hipStream_t);
+                                                     int, int, int, int *, hipStream_t);
 
 hipError_t launch_gauge_map(const int32_t *inv, const int32_t *cells, int n, int C, int32_t *gmap, int32_t *gnext,
                             int32_t *internal, hipStream_t s) {
@@ -498,6 +528,28 @@ hipError_t launch_gauge_gather(const State<R> *st, const int32_t *inv, const int
 }
 template hipError_t launch_gauge_gather<double>(const State<double> *, const int32_t *, const int32_t *, int, double *,
                                                 hipStream_t);
+
+template <class R>
+hipError_t launch_gauge_gather_scen(const State<R> *st, int C, int nscen, const int32_t *inv, const int32_t *cells,
+                                    int n, double *out, hipStream_t s) {
+    if (n == 0 || nscen <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_gauge_gather_scen<R>), dim3(div_up(n, 256), nscen), dim3(256), 0, s, st, C, inv, cells, n,
+                       out);
+    return hipGetLastError();
+}
+template hipError_t launch_gauge_gather_scen<double>(const State<double> *, int, int, const int32_t *,
+                                                     const int32_t *, int, double *, hipStream_t);
+template hipError_t launch_gauge_gather_scen<float>(const State<float> *, int, int, const int32_t *, const int32_t *,
+                                                    int, double *, hipStream_t);
+
+template <class R>
+hipError_t launch_scen_to_internal(const double *in, const int32_t *orig, R *out, int C, int nscen, hipStream_t s) {
+    if (C == 0 || nscen <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_scen_to_internal<R>), dim3(div_up(C, 256), nscen), dim3(256), 0, s, in, orig, out, C);
+    return hipGetLastError();
+}
+template hipError_t launch_scen_to_internal<double>(const double *, const int32_t *, double *, int, int, hipStream_t);
+template hipError_t launch_scen_to_internal<float>(const double *, const int32_t *, float *, int, int, hipStream_t);
 template hipError_t launch_gauge_gather<float>(const State<float> *, const int32_t *, const int32_t *, int, double *,
                                                hipStream_t);
 

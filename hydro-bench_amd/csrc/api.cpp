@@ -116,6 +116,7 @@ struct hydro_mesh {
     int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
+    DevBuf scen_st[2], scen_fnc;   // scenario misfit loops: states [S][C] (ping-pong), FNC [S][C]
     DevBuf gmap, gnext, gcells;   // misfit loops: gauge map [C], chain [n], internal ids [n]
     DevBuf tick, barrier, perr, ext_dev;   // tick: StepClock of the GRAPH mode
     bool need_zb1 = false, need_slcs = false;
@@ -1357,8 +1358,8 @@ static int misfit_backward_t(hydro_mesh *m, int nsteps, int seg, const void *ckp
             HIP_TRY(launch_adjoint_step_fused<R>(f, m->twin.as<uint8_t>(), (const State<R> *)states + (size_t)i * C,
                                                  (const R *)wsums + (size_t)i * 3 * C, L[cur], L[cur ^ 1], g_fnc,
                                                  g_zbc, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
-                                                 n > 0 ? resid + (size_t)(s0 + i) * n : nullptr, 1 + (int)(g % per),
-                                                 (int)(g / per), m->adj_flag.as<int>(), s));
+                                                 n > 0 ? resid + (size_t)(s0 + i) * n : nullptr, n, 1,
+                                                 1 + (int)(g % per), (int)(g / per), m->adj_flag.as<int>(), s));
             cur ^= 1;
         }
     }
@@ -1389,6 +1390,160 @@ int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, 
                                            lam_index, s)
                : misfit_backward_t<float>(m, nsteps, seg, ckpt, n, resid, states, wsums, lam, g_fnc, g_zbc,
                                           lam_index, s);
+}
+
+// ---- scenarios: S Manning fields on one mesh, one launch per step ---------
+// The scenarios start from the mesh's current state and time, share its bed,
+// geometry and gauges, and have their own FNC (fnc [S][C], caller order,
+// converted to the mesh's type) and states.  The mesh's own state and time
+// are left as they were.  Buffers as in the one-field loops with a scenario
+// dimension inside each record: ckpt and states [record][S][C] states, wsums
+// [record][S][3][C], zg and resid [step][S][n], lam [2][S][C], g_fnc and
+// g_zbc [S][C] (internal order).
+extern "C++" template <class R>
+static int scen_setup(hydro_mesh *m, int S, const double *fnc, hipStream_t s) {
+    int rc;
+    const size_t C = (size_t)m->CELL;
+    for (int b = 0; b < 2; b++)
+        if (m->scen_st[b].bytes != (size_t)S * C * sizeof(State<R>) &&
+            (rc = m->scen_st[b].alloc((size_t)S * C * sizeof(State<R>))))
+            return rc;
+    if (m->scen_fnc.bytes != (size_t)S * C * sizeof(R) && (rc = m->scen_fnc.alloc((size_t)S * C * sizeof(R))))
+        return rc;
+    HIP_TRY(launch_scen_to_internal<R>(fnc, m->orig.as<int32_t>(), m->scen_fnc.as<R>(), m->CELL, S, s));
+    return HYDRO_OK;
+}
+
+extern "C++" template <class R>
+static FusedView<R> scen_view(hydro_mesh *m, int cur) {
+    FusedView<R> f = fused_view<R>(m, m->cur);
+    f.Sin = m->scen_st[cur].as<State<R>>();
+    f.Sout = m->scen_st[cur ^ 1].as<State<R>>();
+    f.FNC = m->scen_fnc.as<R>();
+    return f;
+}
+
+extern "C++" template <class R>
+static int scen_forward_t(hydro_mesh *m, int S, const double *fnc, int nsteps, int seg, void *ckpt,
+                          const int32_t *gauges, int n, double *zg, hipStream_t s) {
+    int rc;
+    if ((rc = ensure_internal<R>(m, s)) || (rc = scen_setup<R>(m, S, fnc, s)) ||
+        (rc = gauge_tables(m, gauges, n, s)))
+        return rc;
+    const size_t C = (size_t)m->CELL, rec = (size_t)S * C;
+    for (int k = 0; k < S; k++)
+        HIP_TRY(hipMemcpyAsync(m->scen_st[0].as<State<R>>() + k * C, m->st[m->cur].p, m->st[m->cur].bytes,
+                               hipMemcpyDeviceToDevice, s));
+    const int day0 = m->day, kt0 = m->kt;
+    int cur = 0;
+    for (int k = 0; k < nsteps; k++) {
+        if (ckpt && k % seg == 0)
+            HIP_TRY(hipMemcpyAsync((State<R> *)ckpt + (size_t)(k / seg) * rec, m->scen_st[cur].p, rec * sizeof(State<R>),
+                                   hipMemcpyDeviceToDevice, s));
+        FusedView<R> f = scen_view<R>(m, cur);
+        if (n > 0 && k > 0) {
+            f.gauge_cells = m->gcells.as<int32_t>();
+            f.gauge_n = n;
+            f.gauge_out = zg + (size_t)(k - 1) * S * n;
+        }
+        HIP_TRY((launch_fused_step_scen<R, 0>(f, S, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
+        cur ^= 1;
+        advance(m);
+    }
+    if (n > 0 && nsteps > 0)
+        HIP_TRY(launch_gauge_gather_scen<R>(m->scen_st[cur].as<State<R>>(), m->CELL, S, m->inv.as<int32_t>(), gauges, n,
+                                            zg + (size_t)(nsteps - 1) * S * n, s));
+    m->day = day0;
+    m->kt = kt0;
+    return HYDRO_OK;
+}
+
+extern "C++" template <class R>
+static int scen_backward_t(hydro_mesh *m, int S, const double *fnc, int nsteps, int seg, const void *ckpt, int n,
+                           const double *resid, void *states, void *wsums, void *lam, double *g_fnc, double *g_zbc,
+                           int *lam_index, hipStream_t s) {
+    int rc;
+    if ((rc = ensure_internal<R>(m, s)) || (rc = scen_setup<R>(m, S, fnc, s))) return rc;
+    const int per = m->spd - 1;
+    const int day0 = m->day, kt0 = m->kt;
+    const int64_t g0 = global_step(m);
+    const size_t C = (size_t)m->CELL, rec = (size_t)S * C;
+    State<double> *L[2] = {(State<double> *)lam, (State<double> *)lam + rec};
+    HIP_TRY(hipMemsetAsync(L[0], 0, rec * sizeof(State<double>), s));
+    int cur = 0;
+    const int nseg = (nsteps + seg - 1) / seg;
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int s0 = k * seg;
+        const int len = std::min(seg, nsteps - s0);
+        HIP_TRY(hipMemcpyAsync(m->scen_st[0].p, (const State<R> *)ckpt + (size_t)k * rec, rec * sizeof(State<R>),
+                               hipMemcpyDeviceToDevice, s));
+        m->day = (int)((g0 + s0) / per);
+        m->kt = 1 + (int)((g0 + s0) % per);
+        int sc = 0;
+        for (int i = 0; i < len; i++) {   // the segment again, recording each step's input and flux sums
+            FusedView<R> f = scen_view<R>(m, sc);
+            f.rec_state = (State<R> *)states + (size_t)i * rec;
+            f.rec_wsum = (R *)wsums + (size_t)i * 3 * rec;
+            HIP_TRY((launch_fused_step_scen<R, 0>(f, S, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
+            sc ^= 1;
+            advance(m);
+        }
+        const FusedView<R> f = scen_view<R>(m, 0);
+        for (int i = len - 1; i >= 0; i--) {
+            const int64_t g = g0 + s0 + i;
+            HIP_TRY(launch_adjoint_step_fused<R>(f, m->twin.as<uint8_t>(), (const State<R> *)states + (size_t)i * rec,
+                                                 (const R *)wsums + (size_t)i * 3 * rec, L[cur], L[cur ^ 1], g_fnc,
+                                                 g_zbc, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
+                                                 n > 0 ? resid + (size_t)(s0 + i) * S * n : nullptr, n, S,
+                                                 1 + (int)(g % per), (int)(g / per), m->adj_flag.as<int>(), s));
+            cur ^= 1;
+        }
+    }
+    m->day = day0;
+    m->kt = kt0;
+    *lam_index = cur;
+    return HYDRO_OK;
+}
+
+static int scen_ready(hydro_mesh *m, int S, const double *fnc, int nsteps, int seg) {
+    int rc = adj_ready(m, true);
+    if (rc) return rc;
+    if (S <= 0 || !fnc || nsteps < 0 || seg <= 0) return fail(HYDRO_ERR_INVALID, "bad argument");
+    if (m->tune.scheme != HYDRO_SCHEME_TILES || m->rare)
+        return fail(HYDRO_ERR_UNSUPPORTED, "scenario loops need the TILES scheme and no KLAS 3/5/6/7 or unknown codes");
+    const int per = m->spd - 1;
+    if (per <= 0) return fail(HYDRO_ERR_RANGE, "the mesh has no steps per day");
+    if (nsteps > (int64_t)m->d.NDAYS * per - global_step(m))
+        return fail(HYDRO_ERR_RANGE, "%d steps requested past the NDAYS cap", nsteps);
+    return HYDRO_OK;
+}
+
+int hydro_scen_misfit_forward(hydro_mesh *m, int S, const double *fnc, int nsteps, int seg, void *ckpt,
+                              const int32_t *gauges, int n, double *zg, hipStream_t s) {
+    int rc = scen_ready(m, S, fnc, nsteps, seg);
+    if (rc || (rc = check_gauges(m, gauges, n))) return rc;
+    if (n > 0 && !zg) return fail(HYDRO_ERR_INVALID, "bad argument");
+    return m->d.dtype == HYDRO_F64 ? scen_forward_t<double>(m, S, fnc, nsteps, seg, ckpt, gauges, n, zg, s)
+                                   : scen_forward_t<float>(m, S, fnc, nsteps, seg, ckpt, gauges, n, zg, s);
+}
+
+int hydro_scen_misfit_backward(hydro_mesh *m, int S, const double *fnc, int nsteps, int seg, const void *ckpt,
+                               const int32_t *gauges, int n, const double *resid, void *states, void *wsums,
+                               void *lam, double *g_fnc, double *g_zbc, int *lam_index, hipStream_t s) {
+    int rc = scen_ready(m, S, fnc, nsteps, seg);
+    if (rc || (rc = check_gauges(m, gauges, n))) return rc;
+    if (!ckpt || (n > 0 && !resid) || !states || !wsums || !lam || !g_fnc || !g_zbc || !lam_index)
+        return fail(HYDRO_ERR_INVALID, "bad argument");
+    if ((rc = gauge_tables(m, gauges, n, s))) return rc;
+    if (!m->adj_flag.p) {
+        if ((rc = m->adj_flag.alloc(sizeof(int)))) return rc;
+        HIP_TRY(hipMemset(m->adj_flag.p, 0, sizeof(int)));
+    }
+    return m->d.dtype == HYDRO_F64
+               ? scen_backward_t<double>(m, S, fnc, nsteps, seg, ckpt, n, resid, states, wsums, lam, g_fnc, g_zbc,
+                                         lam_index, s)
+               : scen_backward_t<float>(m, S, fnc, nsteps, seg, ckpt, n, resid, states, wsums, lam, g_fnc, g_zbc,
+                                        lam_index, s);
 }
 
 int hydro_cells_map(hydro_mesh *m, const double *in, double *out, int to_external, hipStream_t s) {

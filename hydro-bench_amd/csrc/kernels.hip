@@ -324,9 +324,12 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
     tile_store<R>(f, L, B, la, ja, pa, fl, t.sa, t.c, t.s, RARE && kind == K_BUNK);
 }
 
-template <class R, int VAR, bool RARE, bool COH = false>
+// SCEN: one of several scenarios of the mesh (k_fused_step_scen): Sin and
+// Sout are the scenario's, and its FNC and recording buffers sit `scen`
+// blocks of C (3C for the flux sums) into the arrays.
+template <class R, int VAR, bool RARE, bool COH = false, bool SCEN = false>
 __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
-                                           State<R> *__restrict__ Sout, int t, int kt, int jt) {
+                                           State<R> *__restrict__ Sout, int t, int kt, int jt, int scen = 0) {
     const int B = f.block, C = f.C, i = threadIdx.x;
     const int p0 = t * f.tile, p = p0 + i;
     const bool live = i < f.tile && p < C;
@@ -347,7 +350,7 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
     // conditional load here would make its wait a wait for everything
     z1 = (f.ZB1 ? f.ZB1 : f.ZBC)[pc];
     area = (VAR == 0) ? f.DTA[pc] : f.AREA[pc];   // variant A: DT / AREA itself
-    fnc = f.FNC[pc];
+    fnc = SCEN ? f.FNC[(size_t)scen * C + pc] : f.FNC[pc];
     pad = f.cell_pad[pc];
     if (VAR == 1) {
 #pragma unroll
@@ -404,10 +407,11 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
         }
     }
     if (f.rec_wsum) {
-        f.rec_state[p] = me;
-        f.rec_wsum[p] = WH;
-        f.rec_wsum[C + p] = WU;
-        f.rec_wsum[2 * C + p] = WV;
+        const size_t so = SCEN ? (size_t)scen * C : 0;
+        f.rec_state[so + p] = me;
+        f.rec_wsum[3 * so + p] = WH;
+        f.rec_wsum[3 * so + C + p] = WU;
+        f.rec_wsum[3 * so + 2 * C + p] = WV;
     }
     const CellOut<R> o =
         (VAR == 0) ? update_with_dta<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV)
@@ -437,6 +441,19 @@ template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt, int jt, int remap) {
     gauge_read(f);
     step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
+}
+
+// Scenarios of one mesh in one launch: workgroup w runs tile w % ntiles of
+// scenario w / ntiles.  The scenarios share the mesh's geometry and tasks and
+// differ in state and Manning field (FNC [S][C]); each reads its own gauges.
+template <class R, int VAR, bool RARE>
+__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_scen(FusedView<R> f, int kt, int jt, int remap) {
+    const int scen = blockIdx.x / f.ntiles, wg = blockIdx.x - scen * f.ntiles;
+    const size_t so = (size_t)scen * f.C;
+    if (f.gauge_out != nullptr && wg == 0)
+        for (int i = threadIdx.x; i < f.gauge_n; i += blockDim.x)
+            f.gauge_out[(size_t)scen * f.gauge_n + i] = (double)f.Sin[so + f.gauge_cells[i]].Z;
+    tile_cells<R, VAR, RARE, false, true>(f, f.Sin + so, f.Sout + so, tile_of(wg, f.ntiles, remap), kt, jt, scen);
 }
 
 // TILES scheme limited to 128 VGPRs: four waves per SIMD (LDS allows four
@@ -767,6 +784,17 @@ hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, i
 }
 
 template <class R, int VAR>
+hipError_t launch_fused_step_scen(const FusedView<R> &f, int nscen, int kt, int jt, int block, int remap,
+                                  hipStream_t s) {
+    if (f.C == 0 || nscen <= 0) return hipSuccess;
+    if (block > kFusedMaxBlock || block != f.block || f.scheme != 0 || f.rare) return hipErrorInvalidValue;
+    FusedView<R> a = f;
+    void *args[] = {&a, &kt, &jt, &remap};
+    return hipLaunchKernel((const void *)k_fused_step_scen<R, VAR, false>, dim3(f.ntiles * nscen), dim3(block), args,
+                           step_lds<R>(f.scheme, block), s);
+}
+
+template <class R, int VAR>
 hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock, int offset, int spd, int block,
                                   int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
@@ -908,6 +936,7 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
 
 #define HYDRO_INSTANTIATE(R, VAR)                                                                        \
     template hipError_t launch_calculate_flux<R, VAR>(const ExtView<R> &, int, int, int, hipStream_t);   \
+    template hipError_t launch_fused_step_scen<R, VAR>(const FusedView<R> &, int, int, int, int, int, hipStream_t);  \
     template hipError_t launch_calculate_flux_tick<R, VAR>(const ExtView<R> &, const StepClock *, int, int, int,  \
                                                            hipStream_t);                                         \
     template hipError_t launch_update_cell<R, VAR>(const ExtView<R> &, int, hipStream_t);               \

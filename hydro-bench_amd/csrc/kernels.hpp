@@ -120,6 +120,9 @@ hipError_t launch_update_cell(const ExtView<R> &m, int block, hipStream_t s);
 template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int xcd_remap, hipStream_t s);
 template <class R, int VAR>
+hipError_t launch_fused_step_scen(const FusedView<R> &f, int nscen, int kt, int jt, int block, int xcd_remap,
+                                  hipStream_t s);
+template <class R, int VAR>
 hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock, int offset, int spd, int block,
                                   int xcd_remap, hipStream_t s);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
@@ -190,7 +193,12 @@ template <class R>
 hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const State<R> *s_in, const R *wsum,
                                      const State<double> *lam_out, State<double> *lam_in, double *g_fnc,
                                      double *g_zbc, const int32_t *gmap, const int32_t *gnext, const double *seed,
-                                     int kt, int jt, int *unsupported, hipStream_t s);
+                                     int nseed, int nscen, int kt, int jt, int *unsupported, hipStream_t s);
+template <class R>
+hipError_t launch_gauge_gather_scen(const State<R> *st, int C, int nscen, const int32_t *inv, const int32_t *cells,
+                                    int n, double *out, hipStream_t s);
+template <class R>
+hipError_t launch_scen_to_internal(const double *in, const int32_t *orig, R *out, int C, int nscen, hipStream_t s);
 hipError_t launch_gauge_map(const int32_t *inv, const int32_t *cells, int n, int C, int32_t *gmap, int32_t *gnext,
                             int32_t *internal, hipStream_t s);
 template <class R>

@@ -125,6 +125,10 @@ SIGNATURES = {
     "hydro_misfit_forward": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "hydro_misfit_backward": (c_int, [c_void_p, c_int, c_int] + [c_void_p] * 2 + [c_int] + [c_void_p] * 6
                               + [POINTER(c_int), c_void_p]),
+    "hydro_scen_misfit_forward": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int,
+                                          c_void_p, c_void_p]),
+    "hydro_scen_misfit_backward": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]
+                                   + [c_void_p] * 6 + [POINTER(c_int), c_void_p]),
     # batches
     "hydro_batch_create": (c_int, [c_void_p, c_int, POINTER(Tune), POINTER(c_void_p)]),
     "hydro_batch_destroy": (c_int, [c_void_p]),

@@ -176,6 +176,84 @@ def run_backward(prob: MisfitProblem, buf: _Buffers, zg, n, zbc, gL=1.0):
     return g_n, g_zbc
 
 
+class _ScenBuffers:
+    """Buffers of the scenario loops: each record holds all S scenarios."""
+
+    def __init__(self, dm: DeviceMesh, S: int, nseg: int, seg: int):
+        torch = _torch()
+        C = dm.host.CELL
+        dev = dm.device
+        tt = torch.float64 if dm.host.dtype == np.float64 else torch.float32
+        self.S, self.C = S, C
+        self.ckpt = torch.empty((nseg + 1, S, C, 4), dtype=tt, device=dev)
+        self.states = torch.empty((seg + 1, S, C, 4), dtype=tt, device=dev)
+        self.wsums = torch.empty((seg, S, 3, C), dtype=tt, device=dev)
+        self.lam = torch.zeros((2, S, C, 4), dtype=torch.float64, device=dev)
+        self.g_fnc = torch.zeros((S, C), dtype=torch.float64, device=dev)
+        self.g_zbc = torch.zeros((S, C), dtype=torch.float64, device=dev)
+
+
+def run_forward_scen(prob: MisfitProblem, ns, zbc, keep_checkpoints=True):
+    """The forward run of S Manning fields at once (``ns`` [S, C], caller
+    order; hydro_scen_misfit_forward: one launch per step for all of them);
+    returns gauge Z [steps, S, G] and the buffers holding the checkpoints."""
+    torch = _torch()
+    dm = prob.dm
+    st = dm.stream()
+    _set_params(dm, ns[0], zbc)
+    H0, Z0, _ = _initial_state(prob, zbc)
+    dm.load_state({"H": H0, "U": prob.U0, "V": prob.V0, "Z": Z0})
+    dm.set_time(0, 1)
+    S = int(ns.shape[0])
+    seg = prob.seg
+    nseg = (prob.steps + seg - 1) // seg
+    buf = _ScenBuffers(dm, S, nseg, seg) if keep_checkpoints else None
+    G = int(prob.gauges.numel())
+    fnc = (9.81 * ns * ns).contiguous()
+    zg = torch.empty((prob.steps, S, G), dtype=torch.float64, device=dm.device)
+    ckpt = _vp(buf.ckpt) if buf is not None else ctypes.c_void_p()
+    _lib.check(dm.lib.hydro_scen_misfit_forward(dm.h, S, _vp(fnc), prob.steps, seg, ckpt, _vp(prob.gauges), G,
+                                                _vp(zg), st))
+    return zg, buf
+
+
+def run_backward_scen(prob: MisfitProblem, buf: _ScenBuffers, zg, ns, zbc, gL=1.0):
+    """Adjoint sweep of run_forward_scen for all S fields at once (one launch
+    per adjoint step); returns (dL_s/dn_s, dL_s/dzbc) [S, C], caller order,
+    where L_s is scenario s's misfit."""
+    torch = _torch()
+    dm = prob.dm
+    lib = dm.lib
+    st = dm.stream()
+    S, C = buf.S, buf.C
+    G = int(prob.gauges.numel())
+    resid = (2.0 * gL * (zg - prob.z_obs[:, None, :])).contiguous()
+    fnc = (9.81 * ns * ns).contiguous()
+    buf.g_fnc.zero_()
+    buf.g_zbc.zero_()
+    idx = ctypes.c_int()
+    _lib.check(lib.hydro_scen_misfit_backward(dm.h, S, _vp(fnc), prob.steps, prob.seg, _vp(buf.ckpt),
+                                              _vp(prob.gauges), G, _vp(resid), _vp(buf.states), _vp(buf.wsums),
+                                              _vp(buf.lam), _vp(buf.g_fnc), _vp(buf.g_zbc), ctypes.byref(idx), st))
+    cur = idx.value
+    bad = ctypes.c_int()
+    _lib.check(lib.hydro_adjoint_check(dm.h, ctypes.byref(bad)))
+    if bad.value:
+        raise _lib.HydroError(_lib.ERR_UNSUPPORTED, "the mesh has slot kinds without an adjoint")
+    _, _, wet = _initial_state(prob, zbc)
+    g_n = torch.empty((S, C), dtype=torch.float64, device=dm.device)
+    g_z = torch.empty_like(g_n)
+    lamH, lamU, lamV, lamZ = (torch.empty(C, dtype=torch.float64, device=dm.device) for _ in range(4))
+    for k in range(S):
+        _lib.check(lib.hydro_cells_map(dm.h, _vp(buf.g_fnc[k]), _vp(g_n[k]), 1, st))
+        _lib.check(lib.hydro_cells_map(dm.h, _vp(buf.g_zbc[k]), _vp(g_z[k]), 1, st))
+        lam_k = buf.lam[cur, k].contiguous()
+        _lib.check(lib.hydro_states_map(dm.h, _vp(lam_k), _vp(lamH), _vp(lamU), _vp(lamV), _vp(lamZ), 1, st))
+        g_z[k] += torch.where(wet, -lamH, lamZ)
+    g_n = g_n * 2.0 * 9.81 * ns
+    return g_n, g_z
+
+
 class GaugeMisfit:
     """torch.autograd.Function wrapper (built lazily so importing needs no torch)."""
 

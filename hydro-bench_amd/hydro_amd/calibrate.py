@@ -5,7 +5,9 @@ Each iteration runs the forward model with checkpoints and the adjoint sweep
 :mod:`hydro_amd.adjoint`), then moves the per-cell Manning n against the
 gradient with a Barzilai–Borwein step, clamped to a physical range.  The bed
 stays fixed.  Hypotheses (different starting fields) are independent
-calibrations of the same problem.
+calibrations of the same problem; :func:`calibrate_many` runs them together,
+each iteration one scenario-batched forward and adjoint sweep for all
+(``hydro_scen_misfit_forward`` / ``hydro_scen_misfit_backward``).
 """
 from __future__ import annotations
 
@@ -15,7 +17,7 @@ from typing import List
 
 import numpy as np
 
-from .adjoint import MisfitProblem, run_backward, run_forward
+from .adjoint import MisfitProblem, run_backward, run_backward_scen, run_forward, run_forward_scen
 
 
 def smooth_field(x, y, seed, mean, std, length, n_modes=24):
@@ -86,3 +88,74 @@ def calibrate(prob: MisfitProblem, n0, zbc, iters: int = 20, n_min: float = 0.00
     out.n = best_n
     out.best = best_loss
     return out
+
+
+class _Descent:
+    """The step rule of :func:`calibrate` for one hypothesis, fed one
+    (loss, gradient) at a time; proposes the next field."""
+
+    def __init__(self, n0, n_min, n_max, first_step, max_growth):
+        self.n = n0.clone()
+        self.out = Calibration(n=self.n)
+        self.prev_n = self.prev_g = None
+        self.alpha = None
+        self.best_loss = self.best_n = self.best_g = None
+        self.lim = (n_min, n_max)
+        self.first_step, self.max_growth = first_step, max_growth
+
+    def feed(self, loss, g):
+        self.out.losses.append(loss)
+        if self.best_loss is not None and loss > self.best_loss:
+            self.alpha *= 0.25
+            self.n = (self.best_n - self.alpha * self.best_g).clamp(*self.lim)
+            self.prev_n = self.prev_g = None
+            return
+        self.best_loss, self.best_n, self.best_g = loss, self.n.clone(), g
+        if self.alpha is None:
+            self.alpha = self.first_step / max(float(g.abs().max()), 1e-300)
+        elif self.prev_g is not None:
+            dn = self.n - self.prev_n
+            dg = g - self.prev_g
+            den = float((dn * dg).sum())
+            if den > 0:
+                self.alpha = min(float((dn * dn).sum()) / den, self.max_growth * self.alpha)
+        self.prev_n, self.prev_g = self.n.clone(), g
+        self.n = (self.n - self.alpha * g).clamp(*self.lim)
+
+    def finish(self, last):
+        self.out.losses.append(last)
+        if last <= self.best_loss:
+            self.best_loss, self.best_n = last, self.n
+        self.out.n = self.best_n
+        self.out.best = self.best_loss
+        return self.out
+
+
+def calibrate_many(prob: MisfitProblem, n0s, zbc, iters: int = 20, n_min: float = 0.005, n_max: float = 0.2,
+                   first_step: float = 2e-3, max_growth: float = 4.0) -> List[Calibration]:
+    """:func:`calibrate` for S starting fields ``n0s`` [S, C] at once: each
+    iteration evaluates every hypothesis's current field in one scenario
+    batch (one launch per step and per adjoint step for all of them), then
+    each takes its own step by the same rule.  Returns one Calibration per
+    hypothesis, the same as :func:`calibrate` on each alone."""
+    import torch
+
+    ds = [_Descent(n0s[k], n_min, n_max, first_step, max_growth) for k in range(n0s.shape[0])]
+    for _ in range(iters):
+        ns = torch.stack([d.n for d in ds]).contiguous()
+        zg, buf = run_forward_scen(prob, ns, zbc)
+        losses = _losses(prob, zg)
+        g_n, _ = run_backward_scen(prob, buf, zg, ns, zbc)
+        del buf
+        for k, d in enumerate(ds):
+            d.feed(losses[k], g_n[k].clone())
+    ns = torch.stack([d.n for d in ds]).contiguous()
+    zg, _ = run_forward_scen(prob, ns, zbc, keep_checkpoints=False)
+    last = _losses(prob, zg)
+    return [d.finish(last[k]) for k, d in enumerate(ds)]
+
+
+def _losses(prob, zg):
+    """Each scenario's misfit, summed over a [steps, G] block as the one-field
+    loss is (same order, same bits)."""
+    return [float(((zg[:, k, :].contiguous() - prob.z_obs) ** 2).sum()) for k in range(zg.shape[1])]

@@ -302,6 +302,21 @@ int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const i
 int hydro_misfit_backward(hydro_mesh *m, int nsteps, int seg, const void *ckpt, const int32_t *gauges, int n,
                           const double *resid, void *states, void *wsums, void *lam, double *g_fnc, double *g_zbc,
                           int *lam_index, hipStream_t stream);
+/* The same loops for S scenarios of the mesh at once (S Manning fields; the
+ * bed, geometry, gauges, start state and time are shared): every step is one
+ * launch over all scenarios' tiles, every adjoint step one launch over all
+ * scenarios' cells.  fnc [S][CELL] is FNC = 9.81 n^2 in the caller's cell
+ * order (double; converted to the mesh's type).  The mesh's own state and
+ * time are left unchanged.  Each record of ckpt and states holds S states
+ * [S][state_bytes], of wsums S flux sums; zg and resid are [step][S][n],
+ * lam [2][S] adjoint states, g_fnc and g_zbc [S][CELL] (internal order).
+ * Needs the TILES scheme and a mesh without KLAS 3/5/6/7 or unknown codes.
+ * No counterpart in the reference (batched calibration hypotheses). */
+int hydro_scen_misfit_forward(hydro_mesh *m, int S, const double *fnc, int nsteps, int seg, void *ckpt,
+                              const int32_t *gauges, int n, double *zg, hipStream_t stream);
+int hydro_scen_misfit_backward(hydro_mesh *m, int S, const double *fnc, int nsteps, int seg, const void *ckpt,
+                               const int32_t *gauges, int n, const double *resid, void *states, void *wsums,
+                               void *lam, double *g_fnc, double *g_zbc, int *lam_index, hipStream_t stream);
 
 /* ---- host-side mesh preparation ------------------------------------------ */
 
