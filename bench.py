@@ -123,18 +123,47 @@ def cpu_reference(host, steps):
     return host.CELL / per_step, per_step, runs
 
 
-def cpu_baseline(host, budget_s):
+def cpu_baseline(host, budget_s, threads=1):
     from oracle import hydro_oracle
 
-    r = hydro_oracle.OracleRun(host)
-    t0 = time.perf_counter()
-    r.step(1)
-    one = time.perf_counter() - t0
-    n = max(1, min(host.max_steps - 1, int(budget_s / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    r.step(n)
-    dt = time.perf_counter() - t0
+    old = hydro_oracle.set_threads(threads)
+    try:
+        r = hydro_oracle.OracleRun(host)
+        t0 = time.perf_counter()
+        r.step(1)
+        one = time.perf_counter() - t0
+        n = max(1, min(host.max_steps - 1, int(budget_s / max(one, 1e-6))))
+        t0 = time.perf_counter()
+        r.step(n)
+        dt = time.perf_counter() - t0
+    finally:
+        hydro_oracle.set_threads(old)
     return host.CELL * n / dt, n, dt
+
+
+def host_cpu():
+    """CPU model, logical CPUs of the machine and CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0))
+    return {"cpu": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def cpu_threads():
+    """Threads for the all-core CPU figure: the usable CPUs, capped by
+    OMP_NUM_THREADS when it is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
 
 
 def _log(msg):
@@ -464,6 +493,13 @@ def main():
         port = {"value": v, "unit": "cell-steps/s", "cores": 1, "kind": "port",
                 "sample": f"oracle (C++ restatement of the reference kernels, g++ -O2), "
                           f"{n} fp64 steps of the same 207,234-cell mesh in {dt:.1f} s, 1 thread"}
+        nt = cpu_threads()
+        if nt > 1:
+            vo, no, dto = cpu_baseline(host64, a.cpu_seconds / 2, threads=nt)
+            port["all_cores"] = {"value": vo, "unit": "cell-steps/s", "cores": nt, "kind": "port",
+                                 "sample": f"the same oracle, slot and cell loops over {nt} OpenMP threads, "
+                                           f"{no} fp64 steps in {dto:.1f} s"}
+        port["host"] = host_cpu()
         out["cpu_baseline"] = port
         try:
             vr, per_step, runs = cpu_reference(host64, 60)
@@ -474,7 +510,7 @@ def main():
                           f"and level edges as walls (its input format has no DT = 0.5 s and no safe boundary "
                           f"series); {per_step * 1e3:.1f} ms/step from runs of 2 and 62 steps "
                           f"({runs[2]:.1f} s, {runs[62]:.1f} s), 1 thread",
-                "port": port}
+                "host": port["host"], "port": port}
         except (OSError, subprocess.SubprocessError) as e:
             print(f"reference CPU solver not timed ({e}); cpu_baseline is the oracle", file=sys.stderr)
     print(json.dumps(out))

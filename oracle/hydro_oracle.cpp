@@ -504,14 +504,22 @@ template <class R> void update_one(View<R> &m, int pos) {
     m.W[pos] = W2;
 }
 
+// Threads of the slot and cell loops (oracle_set_threads; 1 by default).
+// Every slot writes only its own FLUX and KLAS entries and reads states the
+// loop does not write, every cell only its own state, so any thread count
+// gives the same bits.
+int g_threads = 1;
+
 template <class R> void calc_flux_all(const oracle_mesh &om, int kt, int jt) {
     View<R> m(om);
     const int n = 4 * m.CELL;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int idx = 0; idx < n; idx++) flux_slot<R>(m, kt, jt, idx);
 }
 
 template <class R> void update_all(const oracle_mesh &om) {
     View<R> m(om);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int pos = 0; pos < m.CELL; pos++) update_one<R>(m, pos);
 }
 
@@ -522,6 +530,13 @@ int steps_per_day_of(const oracle_mesh &om) { return (int)(om.MDT / om.DT); }
 extern "C" {
 
 int oracle_abi_version(void) { return 1; }
+
+// Threads for the step loops (>= 1); returns the previous count.
+int oracle_set_threads(int n) {
+    const int old = g_threads;
+    g_threads = n < 1 ? 1 : n;
+    return old;
+}
 
 void oracle_calculate_flux(const oracle_mesh *m, int kt, int jt) {
     if (m->dtype == 1) calc_flux_all<float>(*m, kt, jt);

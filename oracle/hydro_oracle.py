@@ -54,8 +54,16 @@ def load():
         lib.oracle_osher.argtypes = [c_int, POINTER(c_double), POINTER(c_double), c_double, c_double,
                                      POINTER(c_double)]
         lib.oracle_precalculate.argtypes = [c_int, c_int] + [c_void_p] * 6 + [c_double] + [c_void_p] * 9
+        lib.oracle_set_threads.argtypes = [c_int]
+        lib.oracle_set_threads.restype = c_int
         _lib = lib
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's slot and cell loops (OpenMP; 1 by default,
+    the same bits at any count).  Returns the previous count."""
+    return load().oracle_set_threads(int(n))
 
 
 MUTABLE = ("H", "U", "V", "Z", "W", "KLAS", "FLUX0", "FLUX1", "FLUX2", "FLUX3")

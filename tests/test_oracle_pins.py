@@ -337,3 +337,22 @@ def test_oracle_step_cap(oracle):
     r = oracle.OracleRun(host)
     with pytest.raises(RuntimeError):
         r.step(host.max_steps + 1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_oracle_threads_same_bits(oracle, dtype):
+    """The slot and cell loops split over threads (the CPU baseline's
+    all-core figure) give the single-thread bits, fluxes included."""
+    host = cached_mesh("channel_6675", dtype=dtype)
+    keys = ("H", "U", "V", "Z", "W", "FLUX0", "FLUX1", "FLUX2", "FLUX3")
+    out = {}
+    try:
+        for t in (1, 4):
+            oracle.set_threads(t)
+            r = oracle.OracleRun(host)
+            r.step(40)
+            out[t] = {k: r[k].copy() for k in keys}
+    finally:
+        oracle.set_threads(1)
+    for k in keys:
+        assert np.array_equal(out[1][k].view(np.uint8), out[4][k].view(np.uint8)), k
