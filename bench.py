@@ -453,7 +453,7 @@ def main():
     if two:
         bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C / 2.0
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-    kernel = {"persistent": "k_persistent<double,0,false,0>", "graph": "k_fused_step_tick<double,0,false,0>",
+    kernel = {"persistent": "k_persistent_tiles<double,0,false>", "graph": "k_fused_step_tick<double,0,false,0>",
               "fused": "k_fused_step<double,0,false,0>", "two_kernel": "k_calculate_flux+k_update_cell",
               "graph_two_kernel": "k_calculate_flux_tick+k_update_cell"}[a.mode]
     traffic = None
