@@ -324,13 +324,30 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
     tile_store<R>(f, L, B, la, ja, pa, fl, t.sa, t.c, t.s, RARE && kind == K_BUNK);
 }
 
+// Launches that start a step only once the previous one has ended have
+// nothing to wait for inside a tile.
+struct NoWait {
+    __device__ bool operator()() const { return true; }
+};
+
 // SCEN: one of several scenarios of the mesh (k_fused_step_scen): Sin and
 // Sout are the scenario's, and its FNC and recording buffers sit `scen`
 // blocks of C (3C for the flux sums) into the arrays.
-template <class R, int VAR, bool RARE, bool COH = false, bool SCEN = false>
-__device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
-                                           State<R> *__restrict__ Sout, int t, int kt, int jt, int scen = 0) {
-    const int B = f.block, C = f.C, i = threadIdx.x;
+// wait(): called by every thread once the tile's own loads are issued and
+// before any neighbour state is read (persistent launches wait there for the
+// neighbouring tiles' previous step); false ends the tile at once, and
+// tile_cells returns false.
+template <class R, int VAR, bool RARE, bool COH = false, bool SCEN = false, class Wait = NoWait>
+__device__ __forceinline__ bool tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
+                                           State<R> *__restrict__ Sout, int t, int kt, int jt, int scen = 0,
+                                           Wait wait = Wait()) {
+    const int B = f.block, C = f.C;
+    // The thread index passes through an empty asm, so nothing derived from
+    // it (LDS and array offsets) is hoisted out of the persistent kernels'
+    // step and tile loops, where it would hold registers across the whole
+    // body (those kernels stay within 128 VGPRs only without such hoisting).
+    int i = threadIdx.x;
+    asm volatile("" : "+v"(i));
     const int p0 = t * f.tile, p = p0 + i;
     const bool live = i < f.tile && p < C;
     const Scalars<R> S = f.S;
@@ -364,6 +381,7 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
     TaskIn<R> t0 = none_task<R>(), t1 = none_task<R>();
     if (kw < kn) t0 = task_load<R>(f, k0 + i);
     if (kw + B < kn) t1 = task_load<R>(f, k0 + i + B);
+    if (!wait()) return false;
     task_neighbour<COH>(f, Sin, t0, pc);
     task_neighbour<COH>(f, Sin, t1, pc);
     if (live) {
@@ -381,7 +399,7 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
         if (tk.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, tk, Sin, B, p0, kt, jt);
     }
     __syncthreads();
-    if (!live) return;
+    if (!live) return true;
     // Phase 3: the cell's four slots in slot order, then its update.
     const R zb1 = f.ZB1 ? L.zb1[i] : (R)(zbp + S.HM1);
     R WH = 0.0, WU = 0.0, WV = 0.0;
@@ -417,6 +435,7 @@ __device__ __forceinline__ void tile_cells(const FusedView<R> &f, const State<R>
         (VAR == 0) ? update_with_dta<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV)
                    : update_math<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV, sd[0], sd[1], sd[2], sd[3]);
     st_state<COH>(Sout, p, C, State<R>{o.H, o.U, o.V, o.Z});   // W = speed(U, V) is formed on the way out
+    return true;
 }
 
 // One step of tile t under either scheme.
@@ -596,8 +615,10 @@ __device__ __forceinline__ void persistent_tiles(const FusedView<R> &f, const Pe
         const State<R> *a = (s & 1) ? f.Sout : f.Sin;
         State<R> *b = (s & 1) ? const_cast<State<R> *>(f.Sin) : f.Sout;
         for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            if (s > 0 && !wait_tiles(f, t, s, pa)) return;
-            tile_cells<R, VAR, RARE, true>(f, a, b, t, kt, day);
+            // the wait sits between the tile's own loads and its neighbours'
+            if (!tile_cells<R, VAR, RARE, true>(f, a, b, t, kt, day, 0,
+                                                [&]() { return s == 0 || wait_tiles(f, t, s, pa); }))
+                return;
             __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are complete
             __syncthreads();
             if (threadIdx.x == 0) __hip_atomic_store(f.tile_flag + t, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
