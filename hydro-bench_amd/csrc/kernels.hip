@@ -10,6 +10,8 @@
 //  k_persistent : the fused step for many steps in one launch, with a
 //      grid-wide barrier between steps (the design of
 //      cuda_impl/F2_driver.cu:34-44, written natively).
+#include <type_traits>
+
 #include "ext_step.hpp"
 #include "kernels.hpp"
 
@@ -342,12 +344,13 @@ __device__ __forceinline__ bool tile_cells(const FusedView<R> &f, const State<R>
                                            State<R> *__restrict__ Sout, int t, int kt, int jt, int scen = 0,
                                            Wait wait = Wait()) {
     const int B = f.block, C = f.C;
-    // The thread index passes through an empty asm, so nothing derived from
-    // it (LDS and array offsets) is hoisted out of the persistent kernels'
-    // step and tile loops, where it would hold registers across the whole
-    // body (those kernels stay within 128 VGPRs only without such hoisting).
+    // In the persistent kernels (the ones that wait) the thread index passes
+    // through an empty asm, so nothing derived from it (LDS and array
+    // offsets) is hoisted out of their step and tile loops, where it would
+    // hold registers across the whole body (they stay within 128 VGPRs only
+    // without such hoisting).  Single-step kernels keep the plain index.
     int i = threadIdx.x;
-    asm volatile("" : "+v"(i));
+    if constexpr (!std::is_same<Wait, NoWait>::value) asm volatile("" : "+v"(i));
     const int p0 = t * f.tile, p = p0 + i;
     const bool live = i < f.tile && p < C;
     const Scalars<R> S = f.S;
