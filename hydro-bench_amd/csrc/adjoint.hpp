@@ -51,8 +51,11 @@ __host__ __device__ inline void osher_bar(double H_pre, double QL1, double QL2, 
     const int K2 = (CA < UA) ? 1 : (UA >= 0.0 && UA < CA) ? 2 : (UA >= -CA && UA < 0.0) ? 3 : 4;
     const bool rsub = (QR1 >= -CR);
     const int K1 = (QL1 < CL && rsub) ? 1 : (QL1 >= CL && rsub) ? 2 : (QL1 < CL && QR1 < -CR) ? 3 : 4;
-    const bool lsup = (K1 == 2 || K1 == 4);
-    const bool rneg = (K1 == 3 || K1 == 4);
+    // The same single bit tests as the forward osher(): the gfx950 backend
+    // of this toolchain mis-lowers the two-case switch that
+    // (K1 == 2 || K1 == 4) becomes.
+    const bool lsup = (K1 & 1) == 0;
+    const bool rneg = K1 >= 3;
     double FIRb = 0, UAb = 0, FIRmb = 0;
     const bool useM = (K2 == 2 || K2 == 3);
     const bool mleft = (K2 == 2);
@@ -190,7 +193,9 @@ __host__ __device__ inline bool slot_flux_bar(int kind, const Scalars<double> &S
             QL2b += Fb[2] * F0;
         } else {
             // F2 = QL1 > 0 ? H1*QL1*QL2 : 0 unless the kind overwrites it
-            const bool f2pre = (kind == K_B1 || kind == K_B3 || kind == K_B5 || kind == K_B7 || kind == K_BUNK);
+            // kinds {B1, B3, B5, B7, BUNK} as one mask test (no multi-case switch)
+            constexpr unsigned kF2Pre = (1u << K_B1) | (1u << K_B3) | (1u << K_B5) | (1u << K_B7) | (1u << K_BUNK);
+            const bool f2pre = (kF2Pre >> kind) & 1u;
             if (f2pre && QL1 > 0) {
                 H1b += Fb[2] * QL1 * QL2;
                 QL1b += Fb[2] * H1 * QL2;

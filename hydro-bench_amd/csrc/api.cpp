@@ -726,7 +726,7 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
     pa.spd = m->spd;
     pa.barrier = m->barrier.as<unsigned>();
     pa.error = m->perr.as<int>();
-    pa.spin_limit = 1u << 24;
+    pa.spin_limit = m->tune.spin_limit > 0 ? (unsigned)m->tune.spin_limit : 1u << 24;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
     HIP_TRY(hipGetLastError());
     if (f.scheme == HYDRO_SCHEME_TILES) HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles, s));
@@ -761,6 +761,7 @@ hydro_tune default_tune() {
     t.waves_per_simd = 0;
     t.scheme = HYDRO_SCHEME_TILES;
     t.tile_cells = 0;
+    t.spin_limit = 0;
     return t;
 }
 
@@ -854,6 +855,7 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
         return fail(HYDRO_ERR_INVALID, "scheme %d", t.scheme);
     if (t.tile_cells != 0 && (t.tile_cells < 64 || t.tile_cells > t.block))
         return fail(HYDRO_ERR_INVALID, "tile_cells %d (0, or 64..block)", t.tile_cells);
+    if (t.spin_limit < 0) return fail(HYDRO_ERR_INVALID, "spin_limit %d", t.spin_limit);
     int tile = t.block;
     if (t.scheme == HYDRO_SCHEME_TILES && t.tile_cells > 0) {
         tile = t.tile_cells;
@@ -1064,10 +1066,12 @@ int hydro_autotune(hydro_mesh *m, int mode, int steps, const hydro_tune *cands, 
         if (hydro_prepare(m, &c[i]) == HYDRO_OK && restore_snapshot(m, snap, day0, kt0, s) == HYDRO_OK &&
             hydro_step_n(m, warm, mode, s) == HYDRO_OK && restore_snapshot(m, snap, day0, kt0, s) == HYDRO_OK &&
             hipEventRecord(e0, s) == hipSuccess && hydro_step_n(m, steps, mode, s) == HYDRO_OK &&
-            hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
+            hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            check_persistent(m, s) == HYDRO_OK) {   // a persistent run that left at its spin limit does not count
             float ms = 0;
             if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) us = ms * 1000.0f / (float)steps;
         }
+        (void)check_persistent(m, s);   // a failed candidate's flag must not fall on the next one
         if (us_per_step) us_per_step[i] = us;
         if (us > 0 && (best_i < 0 || us < best_us)) {
             best_us = us;
@@ -1566,11 +1570,11 @@ int hydro_refresh_statics(hydro_mesh *m, hipStream_t s) {
     if (!m->prepared) return HYDRO_OK;
     if (m->d.dtype == HYDRO_F64) {
         HIP_TRY(launch_refresh_statics<double>(ext_view<double>(m), m->orig.as<int32_t>(), m->ZBC.as<double>(),
-                                               m->AREA.as<double>(), m->FNC.as<double>(),
+                                               m->AREA.as<double>(), m->FNC.as<double>(), m->DTA.as<double>(),
                                                m->need_zb1 ? m->ZB1.as<double>() : nullptr, m->CELL, s));
     } else {
         HIP_TRY(launch_refresh_statics<float>(ext_view<float>(m), m->orig.as<int32_t>(), m->ZBC.as<float>(),
-                                              m->AREA.as<float>(), m->FNC.as<float>(),
+                                              m->AREA.as<float>(), m->FNC.as<float>(), m->DTA.as<float>(),
                                               m->need_zb1 ? m->ZB1.as<float>() : nullptr, m->CELL, s));
     }
     return HYDRO_OK;
@@ -1583,10 +1587,12 @@ int hydro_refresh_statics(hydro_mesh *m, hipStream_t s) {
 struct hydro_batch {
     std::vector<hydro_mesh *> meshes;
     int dtype = 0, variant = 0, block = 256, remap = 1, chunk = 16;
+    unsigned spin_limit = 1u << 24;
     int ntiles = 0;
     bool rare = false;
     int scheme = 0;
     DevBuf views, tile_mesh, tile0, g0, spd, tick, barrier, perr;
+    bool persist_pending = false;   // a persistent launch whose wait flag has not been read back
     std::vector<int32_t> h_tile0;
     hipStream_t cap_stream = nullptr;
     std::map<GraphKey, std::unique_ptr<GraphExec>> graphs;
@@ -1632,6 +1638,23 @@ template <class R> BatchView<R> batch_view(const hydro_batch *b) {
     return v;
 }
 
+// As check_persistent for one mesh: a batch persistent launch whose waits
+// exceeded their spin limit left early, so the members hold partial states.
+// Reported (and the flag cleared) by the next batch call that launches or
+// reads the states.
+int check_batch_persistent(hydro_batch *b, hipStream_t s) {
+    if (!b->persist_pending) return HYDRO_OK;
+    b->persist_pending = false;
+    int err = 0;
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(&err, b->perr.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        HIP_TRY(hipMemset(b->perr.p, 0, sizeof(int)));
+        return fail(HYDRO_ERR_HIP, "a batch persistent launch stopped: a wait between steps exceeded its spin limit");
+    }
+    return HYDRO_OK;
+}
+
 void batch_advance(hydro_batch *b, int n) {
     for (hydro_mesh *m : b->meshes) {
         for (int i = 0; i < n; i++) advance(m);
@@ -1642,7 +1665,9 @@ void batch_advance(hydro_batch *b, int n) {
 
 template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hipStream_t s) {
     std::vector<char> hv;
-    int rc = batch_upload<R>(b, s, hv);
+    int rc = check_batch_persistent(b, s);
+    if (rc) return rc;
+    rc = batch_upload<R>(b, s, hv);
     if (rc) return rc;
     const BatchView<R> v = batch_view<R>(b);
     if (mode == HYDRO_MODE_FUSED) {
@@ -1686,10 +1711,11 @@ template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hip
             pa.day0 = pa.kt0 = pa.spd = 0;   // per-mesh time comes from the batch tables
             pa.barrier = b->barrier.as<unsigned>();
             pa.error = b->perr.as<int>();
-            pa.spin_limit = 1u << 24;
+            pa.spin_limit = b->spin_limit;
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
             HIP_TRY(hipGetLastError());
             HIP_TRY((launch_batch_persistent<R, VAR>(v, pa, b->block, grid, s)));
+            b->persist_pending = true;
         }
     } else {
         return fail(HYDRO_ERR_UNSUPPORTED, "batch mode %d (FUSED, GRAPH or PERSISTENT)", mode);
@@ -1713,6 +1739,7 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
     b->remap = t.xcd_remap;
     b->chunk = t.graph_chunk;
     b->scheme = t.scheme;
+    if (t.spin_limit > 0) b->spin_limit = (unsigned)t.spin_limit;
     if (t.tile_cells == 0) t.tile_cells = t.block;   // members keep whole tiles: the batch balances them
     std::vector<int32_t> tile_mesh, tile0(n);
     for (int i = 0; i < n; i++) {
@@ -1769,6 +1796,8 @@ int hydro_batch_step_n(hydro_batch *b, int nsteps, int mode, hipStream_t stream)
 
 int hydro_batch_sync(hydro_batch *b, hipStream_t stream) {
     if (!b) return fail(HYDRO_ERR_INVALID, "null batch");
+    int rc0 = check_batch_persistent(b, stream);
+    if (rc0) return rc0;
     for (hydro_mesh *m : b->meshes) {
         int rc = hydro_sync(m, stream);
         if (rc) return rc;

@@ -731,13 +731,18 @@ __global__ void k_refresh_meta(const int32_t *KLAS_ext, const int32_t *orig, uin
     meta[e] = (uint8_t)(kind_of_klas(KLAS_ext[4 * orig[p] + j], prim) | (prim ? META_PRIMARY_BIT : 0));
 }
 
+// DTA = DT / AREA is rebuilt with the expression the host used at prepare
+// (1.0 * DT / (1.0 * AREA) in double, rounded to R), so the fused steps keep
+// reading what the update would form from the new AREA.
 template <class R>
-__global__ void k_refresh_statics(ExtView<R> m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *ZB1, int C) {
+__global__ void k_refresh_statics(ExtView<R> m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *DTA, R *ZB1, int C) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= C) return;
     const int o = orig[p];
     ZBC[p] = m.ZBC[o];
-    AREA[p] = m.AREA[o];
+    const R area = m.AREA[o];
+    AREA[p] = area;
+    DTA[p] = (R)((1.0 * (double)m.S.DT) / (1.0 * (double)area));
     FNC[p] = m.FNC[o];
     if (ZB1) ZB1[p] = m.ZB1[o];
 }
@@ -747,16 +752,17 @@ inline int div_up(int a, int b) { return (a + b - 1) / b; }
 }  // namespace
 
 template <class R>
-hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *ZB1, int C,
-                                  hipStream_t s) {
+hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *DTA, R *ZB1,
+                                  int C, hipStream_t s) {
     if (C == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_refresh_statics<R>), dim3(div_up(C, 256)), dim3(256), 0, s, m, orig, ZBC, AREA, FNC, ZB1, C);
+    hipLaunchKernelGGL((k_refresh_statics<R>), dim3(div_up(C, 256)), dim3(256), 0, s, m, orig, ZBC, AREA, FNC, DTA, ZB1,
+                       C);
     return hipGetLastError();
 }
 template hipError_t launch_refresh_statics<double>(const ExtView<double> &, const int32_t *, double *, double *,
-                                                   double *, double *, int, hipStream_t);
+                                                   double *, double *, double *, int, hipStream_t);
 template hipError_t launch_refresh_statics<float>(const ExtView<float> &, const int32_t *, float *, float *, float *,
-                                                  float *, int, hipStream_t);
+                                                  float *, float *, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Launch helpers.

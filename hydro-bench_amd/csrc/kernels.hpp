@@ -177,10 +177,10 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
 template <class R, int VAR>
 hipError_t batch_persistent_max_grid(int block, bool rare, int scheme, int *grid);
 
-// internal statics <- the caller's ZBC AREA FNC (and ZB1 when stored)
+// internal statics <- the caller's ZBC AREA FNC (and ZB1 when stored); DTA rebuilt from AREA
 template <class R>
-hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *ZB1, int C,
-                                  hipStream_t s);
+hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *DTA, R *ZB1,
+                                  int C, hipStream_t s);
 
 // adjoint.hip (fp64, variant A)
 hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, const State<double> *s_in,

@@ -13,7 +13,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_void_p
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhydro_amd.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # hydro_status
 OK, ERR_INVALID, ERR_HIP, ERR_RANGE, ERR_UNSUPPORTED, ERR_NOMEM = range(6)
@@ -67,7 +67,7 @@ class Tune(ctypes.Structure):
     _fields_ = [
         ("block", c_int32), ("reorder", c_int32), ("xcd_remap", c_int32), ("graph_chunk", c_int32),
         ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("waves_per_simd", c_int32),
-        ("scheme", c_int32), ("tile_cells", c_int32),
+        ("scheme", c_int32), ("tile_cells", c_int32), ("spin_limit", c_int32),
     ]
 
 
@@ -80,7 +80,7 @@ def autotune_candidates():
 
 def default_tune(**kw) -> Tune:
     t = Tune(block=256, reorder=REORDER_TILES, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
-             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0)
+             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0, spin_limit=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t

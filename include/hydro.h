@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define HYDRO_ABI_VERSION 2
+#define HYDRO_ABI_VERSION 3
 
 typedef enum hydro_status {
     HYDRO_OK = 0,
@@ -150,7 +150,10 @@ typedef struct hydro_tune {
     int32_t scheme;         /* hydro_scheme */
     int32_t tile_cells;     /* TILES scheme: cells per tile (64..block); 0 = block, or for a mesh
                                with fewer tiles than CUs one tile per CU (at least 64 cells);
-                               batches use block */
+                               batches use block.  With `block` threads per tile this sets the
+                               edges per thread (hydro_plan_stats) */
+    int32_t spin_limit;     /* PERSISTENT modes: sleeps a wait between steps may take before the
+                               launch gives up and reports HYDRO_ERR_HIP (0 = 2^24) */
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -282,7 +285,8 @@ int hydro_cells_map(hydro_mesh *m, const double *in, double *out, int to_externa
 int hydro_states_map(hydro_mesh *m, void *state, double *H, double *U, double *V, double *Z, int to_external,
                      hipStream_t stream);
 /* re-read ZBC, AREA, FNC (and ZB1 when stored) from the caller's arrays after
- * the caller changed them (e.g. a calibration update) */
+ * the caller changed them (e.g. a calibration update); DT / AREA of the fused
+ * steps is rebuilt from the new AREA, so every mode and the adjoint agree */
 int hydro_refresh_statics(hydro_mesh *m, hipStream_t stream);
 /* Whole-run loops of a gauge misfit, issued from the library with no host
  * synchronisation inside (the per-call helpers above check the gauge list on

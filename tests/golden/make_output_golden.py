@@ -18,6 +18,7 @@ Run from the repository root with the reference checkout at /root/reference.
 """
 import importlib.util
 import os
+import tempfile
 import sys
 
 import numpy as np
@@ -80,7 +81,8 @@ def main(out=os.path.join(ROOT, "tests", "golden", "output_frames.npz")):
     for i, s in enumerate(st):
         for k, v in s.items():
             keep[f"state{i}_{k}"] = v
-    tmp = os.path.join(ROOT, "_work", "output_golden")
+    tmpdir = tempfile.TemporaryDirectory()
+    tmp = tmpdir.name
     for name, dt, ndays in CASES:
         d = os.path.join(tmp, name)
         with ref.OutputWriter(d, mesh, dt, ndays) as w:
@@ -91,6 +93,7 @@ def main(out=os.path.join(ROOT, "tests", "golden", "output_frames.npz")):
         for fn in FILES:
             with open(os.path.join(d, fn), "rb") as f:
                 keep[f"{name}_{fn}"] = np.frombuffer(f.read(), dtype=np.uint8)
+    tmpdir.cleanup()
     np.savez_compressed(out, **keep)
     print(out, os.path.getsize(out), "bytes")
 

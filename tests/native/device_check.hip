@@ -1,6 +1,7 @@
 // Device-vs-host check of the timestep's device functions
-// (hydro-bench_amd/csrc/physics.hpp, pow_fixed.hpp): random inputs covering
-// every Osher case, every wet/dry ladder branch and every boundary kind are
+// (hydro-bench_amd/csrc/physics.hpp, pow_fixed.hpp, and the reverse-mode
+// functions of adjoint.hpp): random inputs covering every Osher case, every
+// wet/dry ladder branch and every boundary kind are
 // evaluated by a gfx950 kernel and by the same source compiled for the host,
 // and compared.  It guards against the device compiler lowering a construct
 // differently from its host compile (the host build is itself compared with
@@ -23,14 +24,15 @@
 #include <random>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "physics.hpp"
 
 using namespace hydro;
 
 namespace {
 
-constexpr int NX = 20;   // inputs per case
-constexpr int NO = 5;    // outputs per case
+constexpr int NX = 24;   // inputs per case
+constexpr int NO = 10;   // outputs per case
 constexpr int NDAYS = 2, NHQ = 4;
 
 // Boundary series of the one-cell boundary context every boundary case uses.
@@ -42,6 +44,7 @@ struct BcArrays {
 
 enum Group {
     G_OSHER = 0, G_SLOT64_A, G_SLOT64_B, G_PAIR64, G_UPDATE64, G_SLOT32_A, G_SLOT32_B, G_PAIR32, G_UPDATE32, G_POW,
+    G_OSHER_BAR, G_SLOT_BAR, G_UPDATE_BAR,
     G_COUNT
 };
 
@@ -60,7 +63,8 @@ __host__ __device__ BcCtx<R> make_ctx(const T *QT, const T *DQT, const T *ZT, co
 }
 
 // x: 0-5 own H U V Z ZB1 ZBC | 6-10 nbr H U V Z ZBC | 11 has | 12,13 c s |
-//    14 prev (all four) | 15 SIDE | 16 kt | 17 jt | 18 AREA | 19 FNC
+//    14 prev (all four) | 15 SIDE | 16 kt | 17 jt | 18 AREA | 19 FNC |
+//    20-23 adjoint seeds of the four outputs (reverse-mode groups)
 template <class R, int VAR>
 __host__ __device__ void eval_slot(const double *x, int kind, const BcCtx<R> &bc, double *o) {
     const Scalars<R> S{(R)1.0, (R)0.001, (R)0.01, (R)0.0};
@@ -91,7 +95,38 @@ template <class R, int VAR> __host__ __device__ void eval_update(const double *x
     o[0] = c.H; o[1] = c.U; o[2] = c.V; o[3] = c.Z; o[4] = c.W;
 }
 
+// Reverse of the Osher flux: the seven input adjoints for seeds x[20..23].
+__host__ __device__ void eval_osher_bar(const double *x, double *o) {
+    const double g[4] = {x[20], x[21], x[22], x[23]};
+    double Hb = 0, QL1b = 0, QL2b = 0, QR0b = 0, QR1b = 0, QR2b = 0, FILb = 0;
+    osher_bar(x[0], x[1], x[2], x[6], x[7], x[8], x[1] + 2 * sqrt(9.81 * x[0]), g, Hb, QL1b, QL2b, QR0b, QR1b,
+              QR2b, FILb);
+    o[0] = Hb; o[1] = QL1b; o[2] = QL2b; o[3] = QR0b; o[4] = QR1b; o[5] = QR2b; o[6] = FILb;
+}
+
+// Reverse of one slot flux (kinds with an adjoint: interior, KLAS 10, 1, 4):
+// the adjoints of both sides' H U V Z ZB.
+__host__ __device__ void eval_slot_bar(const double *x, int kind, const BcCtx<double> &bc, double *o) {
+    const Scalars<double> S{1.0, 0.001, 0.01, 0.0};
+    const SideIn me{x[0], x[1], x[2], x[3], x[5]}, nb{x[6], x[7], x[8], x[9], x[10]};
+    const double Fb[4] = {x[20], x[21], x[22], x[23]};
+    SideBar meb, nbb;
+    const bool ok = slot_flux_bar(kind, S, bc, me, x[11] != 0, nb, x[12], x[13], Fb, meb, nbb);
+    o[0] = meb.H; o[1] = meb.U; o[2] = meb.V; o[3] = meb.Z; o[4] = meb.ZB;
+    o[5] = nbb.H; o[6] = nbb.U; o[7] = nbb.V; o[8] = nbb.Z; o[9] = ok ? nbb.ZB : -1.0;
+}
+
+// Reverse of the cell update (variant A): adjoints of the flux sums, the old
+// state, FNC and ZBC for seeds x[20..23] on (H, U, V, Z).
+__host__ __device__ void eval_update_bar(const double *x, double *o) {
+    const Scalars<double> S{1.0, 0.001, 0.01, 0.0};
+    const UpdateBar b = update_bar(S, fmax(x[0], 0.001), x[1], x[2], x[18], x[19], x[6], x[7], x[8], x[20], x[21],
+                                   x[22], x[23]);
+    o[0] = b.WH; o[1] = b.WU; o[2] = b.WV; o[3] = b.H1; o[4] = b.U1; o[5] = b.V1; o[6] = b.FNC; o[7] = b.ZBC;
+}
+
 __host__ __device__ void eval(int g, const double *x, int kind, const BcArrays &A, double *o) {
+    for (int k = 0; k < NO; k++) o[k] = 0;
     switch (g) {
     case G_OSHER: {
         double FO[4];
@@ -120,6 +155,11 @@ __host__ __device__ void eval(int g, const double *x, int kind, const BcArrays &
         if (kind & 1) eval_update<float, 1>(x, o);
         else eval_update<float, 0>(x, o);
         break;
+    case G_OSHER_BAR: eval_osher_bar(x, o); break;
+    case G_SLOT_BAR:
+        eval_slot_bar(x, kind, make_ctx<double>(A.QT, A.DQT, A.ZT, A.DZT, A.ZW, A.QW, A.BF, A.NHQ1, x), o);
+        break;
+    case G_UPDATE_BAR: eval_update_bar(x, o); break;
     default:
         o[0] = pow1p5(x[0]); o[1] = pow_0p33333(x[0]); o[2] = pow1p5(x[1]); o[3] = pow_0p33333(x[1]);
         o[4] = pow_d(x[0], 2.0);
@@ -141,6 +181,7 @@ void gen(int g, int n, uint64_t seed, std::vector<double> &X, std::vector<int> &
     X.assign((size_t)n * NX, 0);
     K.assign(n, 0);
     const int kinds[] = {K_PRIMARY, K_MIRROR, K_B10, K_B1, K_B3, K_B4, K_B5, K_B6, K_B7, K_BUNK};
+    const int adj_kinds[] = {K_PRIMARY, K_MIRROR, K_B10, K_B1, K_B4};   // the kinds with an adjoint
     auto depth = [&]() {
         const double q = U(r);
         if (q < 0.12) return 0.001;
@@ -155,7 +196,8 @@ void gen(int g, int n, uint64_t seed, std::vector<double> &X, std::vector<int> &
             x[1] = std::pow(10.0, -6 + 10 * U(r));
             continue;
         }
-        if (g == G_OSHER) {
+        for (int k = 20; k < 24; k++) x[k] = 2 * (U(r) - 0.5);
+        if (g == G_OSHER || g == G_OSHER_BAR) {
             x[0] = 0.01 + 4 * U(r); x[1] = 20 * (U(r) - 0.5); x[2] = 10 * (U(r) - 0.5);
             x[6] = 0.01 + 4 * U(r); x[7] = 20 * (U(r) - 0.5); x[8] = 10 * (U(r) - 0.5);
             continue;
@@ -184,7 +226,7 @@ void gen(int g, int n, uint64_t seed, std::vector<double> &X, std::vector<int> &
         x[17] = (int)(U(r) * NDAYS);
         x[18] = 50 + 500 * U(r);
         x[19] = 0.0005 + 0.01 * U(r);
-        if (g == G_UPDATE64 || g == G_UPDATE32) {
+        if (g == G_UPDATE64 || g == G_UPDATE32 || g == G_UPDATE_BAR) {
             x[6] = 50 * (U(r) - 0.5);      // WH
             x[7] = 200 * (U(r) - 0.5);     // WU
             x[8] = 200 * (U(r) - 0.5);     // WV
@@ -192,7 +234,7 @@ void gen(int g, int n, uint64_t seed, std::vector<double> &X, std::vector<int> &
             K[i] = i & 1;                  // variant
             continue;
         }
-        const int kind = kinds[i % 10];
+        const int kind = (g == G_SLOT_BAR) ? adj_kinds[i % 5] : kinds[i % 10];
         K[i] = kind;
         x[11] = (kind <= K_MIRROR) ? ((i % 17) ? 1 : 0) : 0;
     }
@@ -281,7 +323,7 @@ extern "C" int dc_run(int g, int n, uint64_t seed, double *max_ulp, int64_t *ine
     // difference there can lose bits to cancellation in the momentum
     // update, so for the update a difference also has to exceed
     // 1e-12 (1 + |H| + |U| + |V|) to count as bad.
-    const bool upd = (g == G_UPDATE64 || g == G_UPDATE32);
+    const bool upd = (g == G_UPDATE64 || g == G_UPDATE32 || g == G_UPDATE_BAR);
     double mx = 0;
     int64_t ninex = 0, nbad = 0;
     size_t rep = 0;
