@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "diag.hpp"
 #include "kernels.hpp"
 #include "prep.hpp"
 
@@ -119,6 +120,7 @@ struct hydro_mesh {
     DevBuf scen_st[2], scen_fnc;   // scenario misfit loops: states [S][C] (ping-pong), FNC [S][C]
     DevBuf gmap, gnext, gcells;   // misfit loops: gauge map [C], chain [n], internal ids [n]
     DevBuf tick, barrier, perr, ext_dev;   // tick: StepClock of the GRAPH mode
+    DevBuf diag_part;   // hydro_diagnostics: one partial record per 1024 cells
     bool need_zb1 = false, need_slcs = false;
     hipStream_t cap_stream = nullptr;
     std::map<GraphKey, std::unique_ptr<GraphExec>> graphs;
@@ -1124,6 +1126,20 @@ int hydro_dump(hydro_mesh *m, const char *path, hipStream_t stream) {
         fwrite(buf.data(), rs, S4, fp);
     }
     fclose(fp);
+    return HYDRO_OK;
+}
+
+int hydro_diagnostics(hydro_mesh *m, double *out, hipStream_t stream) {
+    if (!m || !out) return fail(HYDRO_ERR_INVALID, "null argument");
+    if (!is_device_ptr(out)) return fail(HYDRO_ERR_INVALID, "out is not a device pointer");
+    int rc = hydro_sync(m, stream);
+    if (rc) return rc;
+    const size_t need = (size_t)std::max(1, diag_records(m->CELL)) * kDiagRecord * sizeof(double);
+    if (m->diag_part.bytes < need && (rc = m->diag_part.alloc(need))) return rc;
+    if (m->d.dtype == HYDRO_F64)
+        HIP_TRY(launch_diagnostics<double>(ext_view<double>(m), m->diag_part.as<double>(), out, stream));
+    else
+        HIP_TRY(launch_diagnostics<float>(ext_view<float>(m), m->diag_part.as<double>(), out, stream));
     return HYDRO_OK;
 }
 

@@ -30,6 +30,9 @@ MODES = {
     "persistent": MODE_PERSISTENT,
     "graph_two_kernel": MODE_GRAPH_TWO_KERNEL,
 }
+# hydro_diagnostics record (include/hydro.h HYDRO_DIAG_*)
+DIAG_FIELDS = ("mass", "momentum_x", "momentum_y", "kinetic_energy", "potential_energy", "klas10_inflow",
+               "klas1_inflow", "nonfinite_h", "inf_h", "h_min", "h_max")
 # hydro_reorder
 REORDER_NONE, REORDER_RCM, REORDER_HILBERT, REORDER_TILES = range(4)
 SCHEME_TILES, SCHEME_CELL = range(2)
@@ -129,6 +132,7 @@ SIGNATURES = {
                                           c_void_p, c_void_p]),
     "hydro_scen_misfit_backward": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]
                                    + [c_void_p] * 6 + [POINTER(c_int), c_void_p]),
+    "hydro_diagnostics": (c_int, [c_void_p, c_void_p, c_void_p]),
     # batches
     "hydro_batch_create": (c_int, [c_void_p, c_int, POINTER(Tune), POINTER(c_void_p)]),
     "hydro_batch_destroy": (c_int, [c_void_p]),

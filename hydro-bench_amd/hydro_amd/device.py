@@ -131,6 +131,19 @@ class DeviceMesh:
     def dump(self, path: str):
         _lib.check(self.lib.hydro_dump(self.h, path.encode(), self.stream()))
 
+    def diagnostics(self, out=None):
+        """Device-reduced conservation and health figures of the current state
+        (hydro_diagnostics): a dict keyed by the reference checker's names.
+        With `out` (a float64 device tensor of len(DIAG_FIELDS)) the values
+        stay on the device and `out` is returned instead."""
+        torch = _torch()
+        buf = out if out is not None else torch.empty(len(_lib.DIAG_FIELDS), dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.hydro_diagnostics(self.h, ctypes.c_void_p(buf.data_ptr()), self.stream()))
+        if out is not None:
+            return out
+        vals = buf.cpu().tolist()
+        return dict(zip(_lib.DIAG_FIELDS, vals))
+
     def set_profiling(self, on: bool):
         _lib.check(self.lib.hydro_set_profiling(self.h, 1 if on else 0))
 

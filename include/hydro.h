@@ -228,6 +228,46 @@ int hydro_dump(hydro_mesh *m, const char *path, hipStream_t stream);
 int hydro_set_profiling(hydro_mesh *m, int enable);
 int hydro_last_timing(const hydro_mesh *m, double *total_ms, int *launches, int *steps);
 
+/* ---- diagnostics ---------------------------------------------------------- */
+/* Conservation and health of the current state, reduced on the device with
+ * wavefront (DPP) reductions: the quantities of the reference's alignment
+ * checker (scripts/check_correctness.py:620-655 _conservation_metrics and
+ * :980-990), there computed with Kahan sums on the host.  out is a DEVICE
+ * array of HYDRO_DIAG_N doubles:
+ *   [HYDRO_DIAG_MASS]        sum H * AREA
+ *   [HYDRO_DIAG_MOM_X/Y]     sum H * U * AREA, sum H * V * AREA
+ *   [HYDRO_DIAG_KINETIC]     sum 0.5 * H * (U^2 + V^2) * AREA
+ *   [HYDRO_DIAG_POTENTIAL]   sum 0.5 * 9.81 * H^2 * AREA
+ *   [HYDRO_DIAG_KLAS10_INFLOW] sum over KLAS 10 slots of FLUX0 * SIDE
+ *   [HYDRO_DIAG_KLAS1_INFLOW]  sum over KLAS 1 slots of FLUX0 * SIDE
+ *                            (outward positive, of the caller's FLUX0: the
+ *                            last flux pass of a two-kernel step, or of a
+ *                            reordered step with materialize_flux)
+ *   [HYDRO_DIAG_NONFINITE_H] cells whose H is NaN or infinite
+ *   [HYDRO_DIAG_INF_H]       cells whose H is infinite
+ *   [HYDRO_DIAG_H_MIN/MAX]   range of the finite depths (NaN if none)
+ * Terms are formed in double from the stored values in the checker's order of
+ * operations and summed in double-double, so each sum is the correctly
+ * rounded sum of its terms (up to near-total cancellation), independent of
+ * the workgroup count.  Writes the state back to the caller's arrays first
+ * (as hydro_sync).  The first call allocates a partial-sum buffer of 144
+ * bytes per 1024 cells. */
+enum {
+    HYDRO_DIAG_MASS = 0,
+    HYDRO_DIAG_MOM_X = 1,
+    HYDRO_DIAG_MOM_Y = 2,
+    HYDRO_DIAG_KINETIC = 3,
+    HYDRO_DIAG_POTENTIAL = 4,
+    HYDRO_DIAG_KLAS10_INFLOW = 5,
+    HYDRO_DIAG_KLAS1_INFLOW = 6,
+    HYDRO_DIAG_NONFINITE_H = 7,
+    HYDRO_DIAG_INF_H = 8,
+    HYDRO_DIAG_H_MIN = 9,
+    HYDRO_DIAG_H_MAX = 10,
+    HYDRO_DIAG_N = 11
+};
+int hydro_diagnostics(hydro_mesh *m, double *out, hipStream_t stream);
+
 /* ---- batches of meshes ---------------------------------------------------- */
 /* No counterpart in the reference (it runs one mesh per process): many
  * meshes of uneven size, or many scenarios as separate meshes, advanced in
