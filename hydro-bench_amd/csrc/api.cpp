@@ -1143,6 +1143,26 @@ int hydro_diagnostics(hydro_mesh *m, double *out, hipStream_t stream) {
     return HYDRO_OK;
 }
 
+int hydro_diagnostics_noalloc(hydro_mesh *m, double *out, hipStream_t stream) {
+    if (!m || !out) return fail(HYDRO_ERR_INVALID, "null argument");
+    if (!is_device_ptr(out)) return fail(HYDRO_ERR_INVALID, "out is not a device pointer");
+    const int nrec = diag_records(m->CELL);
+    if (nrec > kDiagScratchRecords)
+        return fail(HYDRO_ERR_INVALID,
+                    "hydro_diagnostics_noalloc: a mesh of %d cells needs %d partial records; its fixed buffer "
+                    "holds %d (meshes of at most %d cells); use hydro_diagnostics",
+                    m->CELL, nrec, kDiagScratchRecords, kDiagScratchRecords * kDiagCellsPerBlock);
+    double *part = nullptr;
+    HIP_TRY(diag_scratch(&part));
+    int rc = hydro_sync(m, stream);
+    if (rc) return rc;
+    if (m->d.dtype == HYDRO_F64)
+        HIP_TRY(launch_diagnostics<double>(ext_view<double>(m), part, out, stream));
+    else
+        HIP_TRY(launch_diagnostics<float>(ext_view<float>(m), part, out, stream));
+    return HYDRO_OK;
+}
+
 int hydro_set_profiling(hydro_mesh *m, int enable) {
     if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
     m->profiling = enable != 0;

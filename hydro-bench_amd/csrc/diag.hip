@@ -15,10 +15,20 @@
 // Every term is formed in double from the stored values, in the checker's
 // order of operations (H * U * a = (H * U) * a, ...), so the result equals
 // the correctly rounded sum of the checker's terms (double-double sums).
+#include <mutex>
+#include <vector>
+
 #include "diag.hpp"
 #include "wave_reduce.hpp"
 
 namespace hydro {
+
+// Fixed partial-sum buffer of the allocation-free entry point: one record
+// per 1024 cells for meshes of up to kDiagScratchRecords * 1024 cells.  Each
+// GPU holds its own copy until the process ends.  Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in for the
+// symbol); it passes the device address diag_scratch() looked up.
+static __device__ __align__(256) double scratch[kDiagScratchRecords * kDiagRecord];
 
 namespace {
 
@@ -153,6 +163,27 @@ template <class R> hipError_t launch_diagnostics(const ExtView<R> &m, double *pa
     }
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kDiagBlock), 0, s, part, nrec, out);
     return hipGetLastError();
+}
+
+hipError_t diag_scratch(double **out) {
+    static std::mutex mu;
+    static std::vector<double *> addr;   // per device, looked up once with that device current
+    int dev = 0, ndev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    if (addr.empty()) {
+        if ((e = hipGetDeviceCount(&ndev)) != hipSuccess) return e;
+        addr.assign(ndev > 0 ? ndev : 1, nullptr);
+    }
+    if (dev < 0 || dev >= (int)addr.size()) return hipErrorInvalidDevice;
+    if (!addr[dev]) {
+        void *p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+        addr[dev] = static_cast<double *>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
 }
 
 template hipError_t launch_diagnostics<double>(const ExtView<double> &, double *, double *, hipStream_t);

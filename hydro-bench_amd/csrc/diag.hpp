@@ -13,6 +13,14 @@ constexpr int kDiagRecord = 2 * kDiagSums + 4;
 // Cells (and their four slots) per workgroup record.
 constexpr int kDiagCellsPerBlock = 1024;
 
+// Records of the allocation-free entry point's fixed buffer (meshes of up to
+// kDiagScratchRecords * kDiagCellsPerBlock = 1,048,576 cells; 144 KiB).
+constexpr int kDiagScratchRecords = 1024;
+
+// Device address of that buffer on the current device (looked up on the
+// first call per device).
+hipError_t diag_scratch(double **out);
+
 // Records the first pass writes for a mesh of C cells.
 int diag_records(int C);
 

@@ -133,6 +133,7 @@ SIGNATURES = {
     "hydro_scen_misfit_backward": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int]
                                    + [c_void_p] * 6 + [POINTER(c_int), c_void_p]),
     "hydro_diagnostics": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "hydro_diagnostics_noalloc": (c_int, [c_void_p, c_void_p, c_void_p]),
     # batches
     "hydro_batch_create": (c_int, [c_void_p, c_int, POINTER(Tune), POINTER(c_void_p)]),
     "hydro_batch_destroy": (c_int, [c_void_p]),

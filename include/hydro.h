@@ -268,6 +268,17 @@ enum {
 };
 int hydro_diagnostics(hydro_mesh *m, double *out, hipStream_t stream);
 
+/* hydro_diagnostics without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and results; the partial sums go to a fixed buffer compiled into
+ * the library (144 KiB per GPU, held until the process ends), so meshes of
+ * more than 1,048,576 cells are rejected with HYDRO_ERR_INVALID.  All calls
+ * on one GPU share that buffer: calls that overlap on one GPU (other streams,
+ * other threads) race and must be ordered by the caller.  Every record a call
+ * reads it has written first, so nothing needs clearing between calls. */
+int hydro_diagnostics_noalloc(hydro_mesh *m, double *out, hipStream_t stream);
+
 /* ---- batches of meshes ---------------------------------------------------- */
 /* No counterpart in the reference (it runs one mesh per process): many
  * meshes of uneven size, or many scenarios as separate meshes, advanced in
