@@ -36,6 +36,16 @@
 #include <cstring>
 #include <cstdio>
 
+// Planted mistakes for the mutation test of the oracle's pins
+// (tests/test_oracle_mutants.py): a build with -DORACLE_MUTANT=k makes
+// mistake k below, each one a plausible slip in restating the reference;
+// the pins must catch every one.  Every other build has ORACLE_MUTANT 0 and
+// none of them.
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#define MUTANT(k) (ORACLE_MUTANT == (k))
+
 extern "C" {
 
 // Host-side mesh view: same fields and meaning as hydro_mesh_desc in
@@ -169,7 +179,7 @@ template <class R> R qd(R ZL, R ZR, R ZB) {
     R DELTA = HS / H0;
     R out;
     if (DELTA <= SIGMA) {
-        out = std::copysign(CM * std::pow((double)H0, 1.5), (double)(ZL - ZR));
+        out = std::copysign(CM * std::pow((double)(MUTANT(9) ? HS : H0), 1.5), (double)(ZL - ZR));
     } else {
         R DH = ZU - ZD;
         if (DH > 0.09) out = std::copysign(FI * HS * sqrt_r<R>(DH), ZL - ZR);
@@ -185,7 +195,8 @@ template <class R> R laqp(R X, const R *A, const R *B, int MS) {
     if (X < A[0]) return B[0];
     if (X > A[MS - 1]) return B[MS - 1];
     for (int i = 0; i < MS - 1; i++)
-        if (X >= A[i] && X <= A[i + 1]) return B[i] + (B[i + 1] - B[i]) / (A[i + 1] - A[i]) * (X - A[i]);
+        if (X >= A[i] && X <= A[i + 1])
+            return MUTANT(10) ? B[i] : B[i] + (B[i + 1] - B[i]) / (A[i + 1] - A[i]) * (X - A[i]);
     return Y;
 }
 
@@ -232,9 +243,9 @@ void bounda(View<R> &m, int kt, int jt, int pos, int idx, R H_pre, R U_pre, R V_
             R Wt = FIL - m.F0[idx] / HB0;
             HB = Wt * Wt / 39.24;
             if (std::fabs(HB0 - HB) <= 0.005) break;
-            HB0 = HB0 * 0.5 + HB * 0.5;
+            HB0 = MUTANT(5) ? HB0 : HB0 * 0.5 + HB * 0.5;
         }
-        m.F1[idx] = (HB <= 1) ? (R)0 : QB2 / HB;
+        m.F1[idx] = (HB <= 1) ? (R)0 : QB2 / (MUTANT(4) ? H_pre : HB);
         m.F2[idx] = 0;
         m.F3[idx] = 4.905 * HB * HB;
     } else if (kl == 3) {
@@ -260,7 +271,7 @@ void bounda(View<R> &m, int kt, int jt, int pos, int idx, R H_pre, R U_pre, R V_
     } else if (kl == 1) {
         R HB1 = m.ZT[row] + m.DZT[row] * (m.variant == 1 ? kt : jt) - m.ZBC[pos];
         HB1 = std::fmax(HB1, m.HM2);
-        R URB = urb_iterate<R>(QL[1], H_pre, HB1, 0.0001);
+        R URB = urb_iterate<R>(MUTANT(6) ? QL[2] : QL[1], H_pre, HB1, 0.0001);
         m.F0[idx] = HB1 * URB;
         m.F1[idx] = m.F0[idx] * URB;
         m.F3[idx] = 4.905 * HB1 * HB1;
@@ -273,7 +284,7 @@ void bounda(View<R> &m, int kt, int jt, int pos, int idx, R H_pre, R U_pre, R V_
         QL[1] = std::fmax(QL[1], (R)0.0);
         m.F0[idx] = H_pre * QL[1];
         m.F1[idx] = m.F0[idx] * QL[1];
-        m.F3[idx] = 4.905 * H_pre * H_pre * (1.0 - m.JL) * (1.0 - m.JL);
+        m.F3[idx] = 4.905 * H_pre * H_pre * (1.0 - m.JL) * (MUTANT(7) ? 1.0 : (1.0 - m.JL));
     } else if (kl == 6) {
         R TOP = m.BF[pos];
         R ZB = m.ZBC[pos];
@@ -282,7 +293,7 @@ void bounda(View<R> &m, int kt, int jt, int pos, int idx, R H_pre, R U_pre, R V_
             m.F0[idx] = 0; m.F1[idx] = 0; m.F2[idx] = 0;
             m.F3[idx] = 4.905 * H_pre * H_pre;
         } else if (Z_pre > TOP && ZC <= TOP) {
-            m.F0[idx] = K::C0 * std::pow((double)(Z_pre - TOP), 1.5);
+            m.F0[idx] = (MUTANT(8) ? K::C1 : K::C0) * std::pow((double)(Z_pre - TOP), 1.5);
             m.F1[idx] = m.F0[idx] * QL[1];
             m.F2[idx] = m.F0[idx] * QL[2];
             m.F3[idx] = 4.905 * std::pow((double)(TOP - ZB), 2);
@@ -320,7 +331,7 @@ void bounda(View<R> &m, int kt, int jt, int pos, int idx, R H_pre, R U_pre, R V_
         R TOP = m.BF[pos];
         R HB = 0;
         if (Z_pre > TOP || ZC > TOP) {
-            m.KLAS[idx] = 0;
+            if (!MUTANT(11)) m.KLAS[idx] = 0;
             R CQ = qd<R>(Z_pre, ZC, TOP);
             R CB = K::BRDTH / m.SIDE[idx];
             m.F0[idx] = CQ * CB;
@@ -366,7 +377,7 @@ template <class R> void flux_slot(View<R> &m, int kt, int jt, int idx) {
     } else if (H1 <= m.HM1 && HC <= m.HM1) {
         F0[idx] = 0; F1[idx] = 0; F2[idx] = 0; F3[idx] = 0;
     } else if (ZI <= BC) {
-        F0[idx] = -K::C1 * std::pow((double)HC, 1.5);
+        F0[idx] = -K::C1 * std::pow((double)(MUTANT(12) ? H1 : HC), 1.5);
         F1[idx] = H1 * QL[1] * std::fabs(QL[1]);
         F2[idx] = 0;
         F3[idx] = 4.905 * H1 * H1;
@@ -375,6 +386,21 @@ template <class R> void flux_slot(View<R> &m, int kt, int jt, int idx) {
         F1[idx] = H1 * std::fabs(QL[1]) * QL[1];
         F2[idx] = H1 * std::fabs(QL[1]) * QL[2];
         F3[idx] = 0;
+    } else if (MUTANT(3) && HC <= m.HM2) {   // the neighbour's shallow test first
+        if (ZI > ZC) {
+            R DH = std::fmax(ZI - BC, m.HM1);
+            R UN = K::C1 * sqrt_r<R>(DH);
+            R HC1 = ZC - ZBp;
+            F0[idx] = DH * UN;
+            F1[idx] = F0[idx] * UN;
+            F2[idx] = F0[idx] * QL[2];
+            F3[idx] = 4.905 * HC1 * HC1;
+        } else {
+            F0[idx] = -K::C1 * std::pow((double)HC, 1.5);
+            F1[idx] = H1 * QL[1] * QL[1];
+            F2[idx] = 0;
+            F3[idx] = 4.905 * H1 * H1;
+        }
     } else if (H1 <= m.HM2) {
         if (ZC > ZI) {
             R DH = std::fmax(ZC - ZBp, m.HM1);
@@ -384,7 +410,7 @@ template <class R> void flux_slot(View<R> &m, int kt, int jt, int idx) {
             F2[idx] = F0[idx] * (VC * COSJ - UC * SINJ);
             F3[idx] = 4.905 * H1 * H1;
         } else {
-            F0[idx] = K::C1 * std::pow((double)H1, 1.5);
+            F0[idx] = K::C1 * std::pow((double)(MUTANT(1) ? HC : H1), 1.5);
             F1[idx] = 0; F2[idx] = 0;
             F3[idx] = 4.905 * H1 * H1;
         }
@@ -392,7 +418,7 @@ template <class R> void flux_slot(View<R> &m, int kt, int jt, int idx) {
         if (ZI > ZC) {
             R DH = std::fmax(ZI - BC, m.HM1);
             R UN = K::C1 * sqrt_r<R>(DH);
-            R HC1 = ZC - ZBp;
+            R HC1 = ZC - (MUTANT(2) ? BC : ZBp);
             F0[idx] = DH * UN;
             F1[idx] = F0[idx] * UN;
             F2[idx] = F0[idx] * QL[2];

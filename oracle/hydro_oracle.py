@@ -41,6 +41,27 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _bind(lib):
+    lib.oracle_calculate_flux.argtypes = [POINTER(OracleMesh), c_int, c_int]
+    lib.oracle_update_cell.argtypes = [POINTER(OracleMesh)]
+    lib.oracle_step_n.argtypes = [POINTER(OracleMesh), c_int, POINTER(c_int), POINTER(c_int)]
+    lib.oracle_step_n.restype = c_int
+    lib.oracle_osher.argtypes = [c_int, POINTER(c_double), POINTER(c_double), c_double, c_double,
+                                 POINTER(c_double)]
+    lib.oracle_precalculate.argtypes = [c_int, c_int] + [c_void_p] * 6 + [c_double] + [c_void_p] * 9
+    lib.oracle_set_threads.argtypes = [c_int]
+    lib.oracle_set_threads.restype = c_int
+    return lib
+
+
+def load_mutant(k, outdir):
+    """The oracle built with planted mistake k (ORACLE_MUTANT, for the
+    mutation test of the pins), compiled into `outdir`."""
+    out = os.path.join(outdir, f"libhydro_oracle_mutant{k}.so")
+    subprocess.run(["make", "-s", "-C", _HERE, "mutant", f"K={k}", f"OUT={out}"], check=True)
+    return _bind(ctypes.CDLL(out))
+
+
 def load():
     global _lib
     if _lib is None:
@@ -72,8 +93,8 @@ MUTABLE = ("H", "U", "V", "Z", "W", "KLAS", "FLUX0", "FLUX1", "FLUX2", "FLUX3")
 class OracleRun:
     """CPU run of one mesh; `variant` 0 = A, 1 = B."""
 
-    def __init__(self, mesh, variant: int = 0):
-        self.lib = load()
+    def __init__(self, mesh, variant: int = 0, lib=None):
+        self.lib = lib if lib is not None else load()
         self.mesh = mesh
         self.a = {k: (v.copy() if k in MUTABLE else v) for k, v in mesh.arrays.items()}
         om = OracleMesh()
