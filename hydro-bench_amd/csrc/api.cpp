@@ -121,6 +121,11 @@ struct hydro_mesh {
     DevBuf gmap, gnext, gcells;   // misfit loops: gauge map [C], chain [n], internal ids [n]
     DevBuf tick, barrier, perr, ext_dev;   // tick: StepClock of the GRAPH mode
     DevBuf diag_part;   // hydro_diagnostics: one partial record per 1024 cells
+    // Real-typed NAC / KLAS of the caller (index_real): int32 copies owned
+    // here serve every kernel (d.NAC / d.KLAS point at them); klas_real is
+    // the caller's Real KLAS, which KLAS 7 rewrites also update
+    DevBuf nac_own, klas_own;
+    void *klas_real = nullptr;
     bool need_zb1 = false, need_slcs = false;
     hipStream_t cap_stream = nullptr;
     std::map<GraphKey, std::unique_ptr<GraphExec>> graphs;
@@ -156,7 +161,8 @@ template <class R> ExtView<R> ext_view(const hydro_mesh *m) {
     v.FNC = (const R *)d.FNC; v.BF = (const R *)d.BoundaryFeature; v.NHQ1 = d.NHQ1;
     v.ZW = (const R *)d.ZW; v.QW = (const R *)d.QW; v.QT = (const R *)d.QT; v.DQT = (const R *)d.DQT;
     v.ZT = (const R *)d.ZT; v.DZT = (const R *)d.DZT;
-    v.NAC = d.NAC; v.KLAS = d.KLAS;
+    v.NAC = (const int32_t *)d.NAC; v.KLAS = (int32_t *)d.KLAS;
+    v.KLASr = (R *)m->klas_real;
     v.SIDE = (const R *)d.SIDE; v.COSF = (const R *)d.COSF; v.SINF = (const R *)d.SINF;
     v.SLCOS = (const R *)d.SLCOS; v.SLSIN = (const R *)d.SLSIN;
     v.F0 = (R *)d.FLUX0; v.F1 = (R *)d.FLUX1; v.F2 = (R *)d.FLUX2; v.F3 = (R *)d.FLUX3;
@@ -232,7 +238,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
     std::vector<int32_t> NAC, KLAS;
     std::vector<R> ZBC, ZB1, AREA, FNC, SIDE, COSF, SINF, SLCOS, SLSIN;
     int rc;
-    if ((rc = d2h(NAC, d.NAC, S4)) || (rc = d2h(KLAS, d.KLAS, S4)) || (rc = d2h(ZBC, d.ZBC, C)) ||
+    if ((rc = d2h(NAC, (const int32_t *)d.NAC, S4)) || (rc = d2h(KLAS, (const int32_t *)d.KLAS, S4)) ||
+        (rc = d2h(ZBC, d.ZBC, C)) ||
         (rc = d2h(ZB1, d.ZB1, C)) || (rc = d2h(AREA, d.AREA, C)) || (rc = d2h(FNC, d.FNC, C)) ||
         (rc = d2h(SIDE, d.SIDE, S4)) || (rc = d2h(COSF, d.COSF, S4)) || (rc = d2h(SINF, d.SINF, S4)) ||
         (rc = d2h(SLCOS, d.SLCOS, S4)) || (rc = d2h(SLSIN, d.SLSIN, S4)))
@@ -528,7 +535,8 @@ template <class R> int ensure_internal(hydro_mesh *m, hipStream_t s) {
     const ExtView<R> ev = ext_view<R>(m);
     m->cur = 0;
     HIP_TRY(launch_gather<R>(ev, m->orig.as<int32_t>(), m->st[0].as<State<R>>(), m->CELL, s));
-    HIP_TRY(launch_refresh_meta(m->d.KLAS, m->orig.as<int32_t>(), m->meta.as<uint8_t>(), m->CELL, s));
+    HIP_TRY(launch_refresh_meta((const int32_t *)m->d.KLAS, m->orig.as<int32_t>(), m->meta.as<uint8_t>(), m->CELL,
+                                s));
     m->int_valid = true;
     return HYDRO_OK;
 }
@@ -767,6 +775,29 @@ hydro_tune default_tune() {
     return t;
 }
 
+// An index array (NAC or KLAS) of n entries as int32: read as it is, or as
+// Real values of the mesh's dtype that must be whole numbers in int32 range.
+int read_index(std::vector<int32_t> &out, const void *p, size_t n, bool real, int dtype, const char *name) {
+    if (!real) return d2h(out, p, n);
+    std::vector<double> v(n);
+    if (dtype == HYDRO_F64) {
+        int rc = d2h(v, p, n);
+        if (rc) return rc;
+    } else {
+        std::vector<float> f;
+        int rc = d2h(f, p, n);
+        if (rc) return rc;
+        for (size_t i = 0; i < n; i++) v[i] = f[i];
+    }
+    out.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        if (!(v[i] >= -2147483648.0 && v[i] <= 2147483647.0) || v[i] != std::floor(v[i]))
+            return fail(HYDRO_ERR_INVALID, "%s[%zu] = %g is not a whole number", name, i, v[i]);
+        out[i] = (int32_t)v[i];
+    }
+    return HYDRO_OK;
+}
+
 int check_ptr(const void *p, const char *name, bool required) {
     if (!p) return required ? fail(HYDRO_ERR_INVALID, "%s is NULL", name) : HYDRO_OK;
     if (!is_device_ptr(p)) return fail(HYDRO_ERR_INVALID, "%s is not a device pointer", name);
@@ -792,6 +823,7 @@ int hydro_mesh_create(const hydro_mesh_desc *desc, hydro_mesh **out) {
     if (d.CELL < 0 || d.CELL > (1 << 28)) return fail(HYDRO_ERR_INVALID, "CELL %d", d.CELL);
     if (d.NDAYS < 1) return fail(HYDRO_ERR_INVALID, "NDAYS %d", d.NDAYS);
     if (d.NHQ < 0) return fail(HYDRO_ERR_INVALID, "NHQ %d", d.NHQ);
+    if (d.index_real & ~(HYDRO_NAC_REAL | HYDRO_KLAS_REAL)) return fail(HYDRO_ERR_INVALID, "index_real %d", d.index_real);
     if (!(d.DT > 0) || d.MDT <= 0) return fail(HYDRO_ERR_INVALID, "DT %g MDT %d", d.DT, d.MDT);
     if (!(d.HM1 > 0) || !(d.HM2 >= d.HM1)) return fail(HYDRO_ERR_INVALID, "HM1 %g HM2 %g", d.HM1, d.HM2);
     int rc;
@@ -812,10 +844,22 @@ int hydro_mesh_create(const hydro_mesh_desc *desc, hydro_mesh **out) {
     m->rsize = d.dtype == HYDRO_F64 ? 8 : 4;
     m->spd = (int)(d.MDT / d.DT);
     m->tune = default_tune();
-    // Host-side validation of the index arrays the kernels dereference.
+    // Host-side validation of the index arrays the kernels dereference
+    // (Real-typed ones are converted to owned int32 copies here).
     const size_t S4 = 4 * (size_t)d.CELL;
     std::vector<int32_t> NAC, KLAS;
-    if ((rc = d2h(NAC, d.NAC, S4)) || (rc = d2h(KLAS, d.KLAS, S4))) return rc;
+    if ((rc = read_index(NAC, d.NAC, S4, (d.index_real & HYDRO_NAC_REAL) != 0, d.dtype, "NAC")) ||
+        (rc = read_index(KLAS, d.KLAS, S4, (d.index_real & HYDRO_KLAS_REAL) != 0, d.dtype, "KLAS")))
+        return rc;
+    if (d.index_real & HYDRO_NAC_REAL) {
+        if ((rc = h2d(m->nac_own, NAC))) return rc;
+        m->d.NAC = m->nac_own.p;
+    }
+    if (d.index_real & HYDRO_KLAS_REAL) {
+        if ((rc = h2d(m->klas_own, KLAS))) return rc;
+        m->klas_real = d.KLAS;
+        m->d.KLAS = m->klas_own.p;
+    }
     for (size_t s = 0; s < S4; s++) {
         if (NAC[s] < 0 || NAC[s] > d.CELL)
             return fail(HYDRO_ERR_INVALID, "NAC[%zu] = %d outside [0, CELL=%d]", s, NAC[s], d.CELL);
@@ -983,17 +1027,17 @@ namespace {
 // Copies of the caller's mutable arrays (state, W, KLAS, FLUX0..3), so every
 // candidate starts from the same step.
 struct Snapshot {
-    DevBuf buf[10];
-    void *src[10] = {};
-    size_t bytes[10] = {};
+    DevBuf buf[11];
+    void *src[11] = {};
+    size_t bytes[11] = {};
     int n = 0;
     int take(hydro_mesh *m, hipStream_t s) {
         const size_t rs = m->d.dtype == HYDRO_F64 ? 8 : 4, C = (size_t)m->CELL;
-        void *p[10] = {m->d.H, m->d.U, m->d.V, m->d.Z, m->d.W, m->d.KLAS, m->d.FLUX0, m->d.FLUX1, m->d.FLUX2,
-                       m->d.FLUX3};
-        const size_t b[10] = {C * rs, C * rs, C * rs, C * rs, C * rs, 4 * C * 4, 4 * C * rs, 4 * C * rs, 4 * C * rs,
-                              4 * C * rs};
-        for (int i = 0; i < 10; i++) {
+        void *p[11] = {m->d.H, m->d.U, m->d.V, m->d.Z, m->d.W, m->d.KLAS, m->d.FLUX0, m->d.FLUX1, m->d.FLUX2,
+                       m->d.FLUX3, m->klas_real};
+        const size_t b[11] = {C * rs, C * rs, C * rs, C * rs, C * rs, 4 * C * 4, 4 * C * rs, 4 * C * rs, 4 * C * rs,
+                              4 * C * rs, 4 * C * rs};
+        for (int i = 0; i < 11; i++) {
             if (!p[i]) continue;
             int rc = buf[n].alloc(b[i]);
             if (rc) return rc;

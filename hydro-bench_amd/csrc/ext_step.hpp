@@ -49,7 +49,10 @@ __host__ __device__ inline void ext_flux_slot(const ExtView<R> &m, int idx, int 
     bool fired = false;
     const Flux<R> f = slot_flux<R, VAR>(kind, m.S, bc, H1, U1, V1, Z1, ZB1, ZBp, has, Hn, Un, Vn,
                                         Zn, ZBn, m.COSF[idx], m.SINF[idx], prev, &fired);
-    if (fired) m.KLAS[idx] = 0;
+    if (fired) {
+        m.KLAS[idx] = 0;
+        if (m.KLASr) m.KLASr[idx] = 0;
+    }
     m.F0[idx] = f.f0;
     m.F1[idx] = f.f1;
     m.F2[idx] = f.f2;

@@ -101,6 +101,7 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
                 f.meta[e] = (mt & META_PRIMARY_BIT) ? (uint8_t)(K_PRIMARY | META_PRIMARY_BIT)
                                                     : (uint8_t)K_MIRROR;
                 f.ext->KLAS[4 * f.orig[p] + j] = 0;
+                if (f.ext->KLASr) f.ext->KLASr[4 * f.orig[p] + j] = 0;
             }
             const R SLCA = f.SLCOS ? f.SLCOS[e] : SIDE * c;
             const R SLSA = f.SLSIN ? f.SLSIN[e] : SIDE * s;
@@ -322,6 +323,7 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
     if (RARE && fired) {
         f.meta[ea] = (mt & META_PRIMARY_BIT) ? (uint8_t)(K_PRIMARY | META_PRIMARY_BIT) : (uint8_t)K_MIRROR;
         f.ext->KLAS[4 * f.orig[pa] + ja] = 0;
+        if (f.ext->KLASr) f.ext->KLASr[4 * f.orig[pa] + ja] = 0;
     }
     tile_store<R>(f, L, B, la, ja, pa, fl, t.sa, t.c, t.s, RARE && kind == K_BUNK);
 }

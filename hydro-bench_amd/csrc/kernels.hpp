@@ -16,6 +16,7 @@ template <class R> struct ExtView {
     const R *ZW, *QW, *QT, *DQT, *ZT, *DZT;
     const int32_t *NAC;
     int32_t *KLAS;
+    R *KLASr;   // the caller's KLAS when it holds Real values (KLAS 7 rewrites go to both), else null
     const R *SIDE, *COSF, *SINF, *SLCOS, *SLSIN;
     R *F0, *F1, *F2, *F3;
 };

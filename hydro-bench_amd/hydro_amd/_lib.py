@@ -21,6 +21,8 @@ OK, ERR_INVALID, ERR_HIP, ERR_RANGE, ERR_UNSUPPORTED, ERR_NOMEM = range(6)
 F64, F32 = 0, 1
 # hydro_variant
 VARIANT_A, VARIANT_B = 0, 1
+# hydro_mesh_desc.index_real bits
+NAC_REAL, KLAS_REAL = 1, 2
 # hydro_mode
 MODE_TWO_KERNEL, MODE_FUSED, MODE_GRAPH, MODE_PERSISTENT, MODE_GRAPH_TWO_KERNEL = range(5)
 MODES = {
@@ -51,7 +53,7 @@ class MeshDesc(ctypes.Structure):
 
     _fields_ = [
         ("abi_version", c_int32), ("dtype", c_int32), ("variant", c_int32), ("CELL", c_int32),
-        ("NDAYS", c_int32), ("NHQ", c_int32), ("MDT", c_int32), ("reserved0", c_int32),
+        ("NDAYS", c_int32), ("NHQ", c_int32), ("MDT", c_int32), ("index_real", c_int32),
         ("DT", c_double), ("HM1", c_double), ("HM2", c_double), ("JL", c_double),
         ("H", c_void_p), ("U", c_void_p), ("V", c_void_p), ("Z", c_void_p), ("W", c_void_p),
         ("ZBC", c_void_p), ("ZB1", c_void_p), ("AREA", c_void_p), ("FNC", c_void_p),

@@ -40,7 +40,7 @@ class DeviceMesh:
     """
 
     def __init__(self, host: HostMesh, variant: int = _lib.VARIANT_A, device: int = 0, tune=None,
-                 prepare: bool = True):
+                 prepare: bool = True, index_real: int = 0):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("DeviceMesh needs a HIP device; none is visible")
@@ -49,15 +49,19 @@ class DeviceMesh:
         self.device = torch.device("cuda", device)
         self.variant = variant
         self.t = {}
+        real_idx = {"NAC": bool(index_real & _lib.NAC_REAL), "KLAS": bool(index_real & _lib.KLAS_REAL)}
         for k, v in host.arrays.items():
             if k in ("NAP", "XP", "YP", "XYMIN"):
                 continue
+            if real_idx.get(k):   # NAC / KLAS as Real, as the CUDA SideView holds them
+                v = v.astype(host.dtype)
             self.t[k] = torch.from_numpy(np.ascontiguousarray(v)).to(self.device)
         d = _lib.MeshDesc()
         d.abi_version = _lib.ABI_VERSION
         d.dtype = _lib.F64 if host.dtype == np.float64 else _lib.F32
         d.variant = variant
         d.CELL, d.NDAYS, d.NHQ, d.MDT = host.CELL, host.NDAYS, host.NHQ, host.MDT
+        d.index_real = int(index_real)
         d.DT, d.HM1, d.HM2, d.JL = host.DT, host.HM1, host.HM2, host.JL
         for name, _t in d._fields_:
             if name in self.t:
@@ -169,28 +173,73 @@ class DeviceMesh:
         return {k: self.t[k].cpu().numpy() for k in names}
 
 
-def run(mesh: HostMesh, steps: Optional[int] = None, mode: str = "two_kernel", variant: int = _lib.VARIANT_A,
-        device: int = 0, tune=None):
-    """The reference's Python contract (taichi_impl/F2_hydro_taichi_fp64.py:29,773):
+# Mesh names of the reference's Taichi loader (taichi_impl/mesh_loader_f2.py:61-90:
+# "default" is its 24,020-cell mesh, "20w" its 207,234-cell one), mapped to the
+# seeded synthetic meshes of the same sizes; a directory is read as a .DAT mesh.
+MESH_NAMES = {"default": "floodplain_24020", "20w": "floodplain_207234"}
+BACKENDS = ("cuda", "hip", "rocm", "gpu")
+
+
+def _mesh_of(mesh, dtype):
+    import os
+
+    from . import mesh as M
+
+    if isinstance(mesh, HostMesh):
+        return mesh
+    if mesh in MESH_NAMES:
+        return M.make_config(MESH_NAMES[mesh], dtype=dtype)
+    if mesh in M.CONFIGS:
+        return M.make_config(mesh, dtype=dtype)
+    if isinstance(mesh, str) and os.path.isdir(mesh):
+        from .dat import load_dat
+
+        return load_dat(mesh, dtype=dtype)
+    raise ValueError(f"unknown mesh {mesh!r}: a HostMesh, one of {sorted(MESH_NAMES) + sorted(M.CONFIGS)}, "
+                     f"or a .DAT directory")
+
+
+def run(days: int = 10, backend: str = "cuda", mesh="default", steps: Optional[int] = None, *,
+        mode: str = "two_kernel", variant: int = _lib.VARIANT_A, dtype=np.float64, device: int = 0, tune=None,
+        index_real: int = 0):
+    """The reference's Python entry point, ``run(days, backend, mesh, steps)``
+    (taichi_impl/F2_hydro_taichi_fp64.py:29, step loop :734-753, return :773):
 
     returns ``(step_fn, sync_fn, H, U, V, Z, FLUX0, FLUX1, FLUX2, FLUX3)``.
-    ``step_fn(on_step=None)`` runs ``steps`` steps over the (day, kt) grid
-    (T2:734-753) and, when ``on_step`` is given, calls ``on_step(s)`` after
-    each step once the state has been synchronised (s counts from 1).
-    The returned tensors are the live device arrays; in reordered modes they
-    are refreshed by ``sync_fn()`` (and before every ``on_step``).
+
+    * ``backend``: "cuda" (or "hip", "rocm", "gpu") runs on ``cuda:<device>``;
+      there is no CPU backend (the reference's "cpu" runs Taichi on the host).
+    * ``mesh``: a HostMesh, "default" / "20w" (the reference loader's names,
+      as seeded synthetic meshes of its 24,020 and 207,234 cells), a workload
+      name of hydro_amd.CONFIGS, or a directory in the reference's .DAT format.
+    * ``step_fn(on_step=None)`` runs, from day 0, the steps of
+      ``min(days, NDAYS)`` days (``steps_per_day - 1`` each) or, when
+      ``steps`` is given, ``steps`` steps over up to NDAYS days, stopping at the
+      NDAYS cap as the reference's loop does; with ``on_step``, the state is
+      synchronised after every step and ``on_step(s)`` called (s from 1).
+      Every call starts again at day 0, kt 1, from the state it finds.
+    * ``sync_fn()`` makes the returned tensors current (reordered modes keep
+      the state in the library's layout until then) and waits for the device.
+
+    The returned tensors are the live device arrays in the reference layout.
+    ``mode`` picks the execution mode (the reference runs the two-kernel step).
     """
-    dm = DeviceMesh(mesh, variant=variant, device=device, tune=tune,
-                    prepare=(mode not in ("two_kernel", "graph_two_kernel")))
-    total = mesh.max_steps if steps is None else int(steps)
-    if total > mesh.max_steps:
-        raise _lib.HydroError(_lib.ERR_RANGE, f"{total} steps exceed the NDAYS cap {mesh.max_steps}")
+    if backend not in BACKENDS:
+        raise ValueError(f"backend {backend!r}: this library runs on the GPU only ({', '.join(BACKENDS)})")
+    host = _mesh_of(mesh, dtype)
+    dm = DeviceMesh(host, variant=variant, device=device, tune=tune,
+                    prepare=(mode not in ("two_kernel", "graph_two_kernel")), index_real=index_real)
+    per_day = max(host.steps_per_day - 1, 1)
+    total = int(steps) if steps is not None else per_day * int(days)
+    day_limit = host.NDAYS if steps is not None else min(int(days), host.NDAYS)
+    total = max(0, min(total, day_limit * per_day))
 
     def sync_fn():
         dm.sync()
         _torch().cuda.synchronize(dm.device)
 
     def step_fn(on_step: Optional[Callable[[int], None]] = None):
+        dm.set_time(0, 1)
         if on_step is None:
             dm.step(total, mode)
             return
@@ -200,5 +249,6 @@ def run(mesh: HostMesh, steps: Optional[int] = None, mode: str = "two_kernel", v
             on_step(s)
 
     step_fn.mesh = dm
+    step_fn.total_steps = total
     t = dm.t
     return step_fn, sync_fn, t["H"], t["U"], t["V"], t["Z"], t["FLUX0"], t["FLUX1"], t["FLUX2"], t["FLUX3"]

@@ -461,10 +461,19 @@ def _ensemble_member(args):
     return make_floodplain_mesh(steps=steps, DT=0.5, dtype=dtype, **spec)
 
 
-def make_ensemble(seed=3003, n=128, lo=2000, hi=207234, steps=500, dtype=np.float32, workers=None):
-    """The members of the batched ensemble, built in parallel processes."""
+def make_ensemble(seed=3003, n=128, lo=2000, hi=207234, steps=500, dtype=np.float32, workers=None, threads=None):
+    """The members of the batched ensemble, built in parallel processes
+    (`workers`, forked: only before the process uses the GPU) or threads of
+    this process (`threads`: safe at any time; the builders release the
+    interpreter lock in their array work).  Every member is built from its
+    own seed, so the result does not depend on either."""
     specs = ensemble_specs(seed, n, lo, hi)
     jobs = [(sp, steps, dtype) for sp in specs]
+    if threads:
+        from multiprocessing.pool import ThreadPool
+
+        with ThreadPool(int(threads)) as pool:
+            return pool.map(_ensemble_member, jobs, chunksize=1)
     workers = workers or min(16, os.cpu_count() or 1)
     if workers <= 1 or n == 1:
         return [_ensemble_member(j) for j in jobs]

@@ -31,9 +31,10 @@
  *   time series    [NDAYS*CELL]      QT DQT ZT DZT, element [jt*CELL + cell]
  *   rating curves  [CELL*NHQ]        ZW QW, element [cell*NHQ + k]
  * Real arrays are double when dtype == HYDRO_F64 and float when HYDRO_F32.
- * NAC and KLAS are int32 here; the CUDA reference stores them as Real, the
- * Taichi port stores NAC as i32 (F2_hydro_taichi_fp64.py:46).  Their values
- * are small integers, so every comparison the kernels make is unchanged.
+ * NAC and KLAS are int32, or Real as the CUDA reference stores them
+ * (hydro_mesh_desc.index_real).  Their values are small integers, so every
+ * comparison the kernels make is unchanged; Real ones are converted once, at
+ * create, into int32 copies the library owns, and KLAS 7 rewrites go to both.
  *
  * Threading: calls on one mesh are stream ordered and must not overlap
  * (H, U, V, Z are updated in place and KLAS 7 slots are rewritten), exactly as
@@ -95,7 +96,10 @@ typedef struct hydro_mesh_desc {
     int32_t NDAYS;
     int32_t NHQ;
     int32_t MDT;           /* seconds per boundary-series row; steps_per_day = (int)(MDT / DT) */
-    int32_t reserved0;
+    int32_t index_real;    /* element type of NAC and KLAS: 0 = int32 (both); HYDRO_NAC_REAL and / or
+                              HYDRO_KLAS_REAL = Real (double / float by dtype), as the CUDA SideView
+                              holds both (side.hpp:29-43) and the Taichi port KLAS (f64, NAC i32,
+                              F2_hydro_taichi_fp64.py:46-47).  Real values must be whole numbers. */
     double DT, HM1, HM2, JL; /* converted to Real inside (float for HYDRO_F32) */
 
     /* state [CELL], read and written */
@@ -108,11 +112,15 @@ typedef struct hydro_mesh_desc {
     /* boundary series [NDAYS*CELL] */
     const void *QT, *DQT, *ZT, *DZT;
     /* slots [4*CELL] */
-    const int32_t *NAC;
-    int32_t *KLAS;         /* written: a KLAS 7 slot that overflows becomes 0 */
+    const void *NAC;       /* int32, or Real with HYDRO_NAC_REAL */
+    void *KLAS;            /* int32, or Real with HYDRO_KLAS_REAL; written: a KLAS 7 slot that
+                              overflows becomes 0 */
     const void *SIDE, *COSF, *SINF, *SLCOS, *SLSIN;
     void *FLUX0, *FLUX1, *FLUX2, *FLUX3; /* written by calculate_flux */
 } hydro_mesh_desc;
+
+#define HYDRO_NAC_REAL 1
+#define HYDRO_KLAS_REAL 2
 
 typedef struct hydro_mesh hydro_mesh; /* opaque handle */
 

@@ -46,12 +46,15 @@ def test_create_rejects_bad_descriptors():
     # host memory is not accepted where device pointers are required
     buf = np.zeros(64)
     for name, _ in d._fields_:
-        if name not in ("abi_version", "dtype", "variant", "CELL", "NDAYS", "NHQ", "MDT", "reserved0",
+        if name not in ("abi_version", "dtype", "variant", "CELL", "NDAYS", "NHQ", "MDT", "index_real",
                         "DT", "HM1", "HM2", "JL"):
             setattr(d, name, buf.ctypes.data)
     assert lib.hydro_mesh_create(ctypes.byref(d), ctypes.byref(h)) == L.ERR_INVALID
     assert b"device pointer" in lib.hydro_last_error()
     assert not h.value
+    d.index_real = 4                      # only the NAC and KLAS bits exist
+    assert lib.hydro_mesh_create(ctypes.byref(d), ctypes.byref(h)) == L.ERR_INVALID
+    assert b"index_real" in lib.hydro_last_error()
 
 
 def test_null_handle_calls_fail_cleanly():
