@@ -21,7 +21,8 @@ def test_more_than_one_gpu_is_refused():
 @pytest.mark.gpu
 def test_headline_line_fields():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "20", "--warmup", "2",
-                        "--no-cpu", "--no-fp32"], cwd=ROOT, capture_output=True, text=True, timeout=300)
+                        "--no-cpu", "--no-fp32", "--no-extras"], cwd=ROOT, capture_output=True, text=True,
+                       timeout=300)
     assert p.returncode == 0, p.stderr[-2000:]
     lines = [l for l in p.stdout.splitlines() if l.strip()]
     assert len(lines) == 1
