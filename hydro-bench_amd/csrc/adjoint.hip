@@ -150,25 +150,38 @@ __global__ void k_adj_flux(FusedView<double> f, const uint8_t *twin, const State
     if (bad) *unsupported = 1;
 }
 
-// Adjoint step in one launch.  A workgroup covers kAdjCells consecutive
-// cells with eight threads per cell: threads 32w .. 32w+31 take slot
-// j = w & 3 of those cells, the own part (w < 4, this cell's slot flux) or
-// the twin part (w >= 4, the neighbour's slot facing this cell), so every
-// thread reverses one slot flux and the two kinds of work sit in different
-// wavefronts.
-//   1. the first own group reverses its cells' update (flux-sum adjoints to
-//      LDS); each twin thread reverses the update of its slot's neighbour
-//      itself (the twin slot's seed), which replaces a separate update pass
-//      and a grid-wide dependency;
-//   2. each thread reverses its slot flux for this cell's part and leaves it
-//      in LDS;
-//   3. the first group adds them to its cell's update adjoint in slot order,
-//      own before twin, the order k_adj_flux uses, so both paths give the
-//      same bits.
-// Gauge seeds enter where lam_out is read: gmap[p] is the first gauge index
-// on cell p (-1 none), gnext chains repeated ids in list order, and
-// seed[i] is added to the Z adjoint in that order (as k_gauge_seed adds it).
-constexpr int kAdjCells = 32;
+// Adjoint step in one launch.  A workgroup takes kAdjTile consecutive cells
+// of the layout (whose tiles keep neighbours close) with eight threads per
+// cell: thread (l, j) for slot j of cell l, and twin thread (l, j) for the
+// neighbour's slot facing it.
+//   1. thread (l, j) reverses its slot's flux (seeded from the flux-sum
+//      adjoints of its cell, ub_cur: the reversed cell update of every cell
+//      for this step), which gives that flux's adjoint with respect to both
+//      cells' states: the own part stays with slot j of this cell; the
+//      neighbour's part is the twin part of the neighbour's slot facing this
+//      cell, handed over in LDS when the neighbour is among the
+//      workgroup's cells;
+//   2. for a neighbour outside them, twin thread (l, j) reverses the
+//      neighbour's facing slot (seeded from the neighbour's ub_cur) for the
+//      twin part;
+//   3. thread (l, 0) adds the parts to its cell's update adjoint in slot
+//      order, own before twin, the order k_adj_flux uses; both halves of a
+//      slot's reversal are the same call's results there and here, so both
+//      paths give the same bits;
+//   4. with a previous step in the same recorded segment, that thread then
+//      reverses its cell's update of the previous step (whose output adjoint
+//      it has just formed) into ub_next, so every cell's update is reversed
+//      once per step and the next launch's slots read it.
+// k_adj_prep fills ub for the last step of a segment.
+// Gauge seeds enter where an update's output adjoint is formed: gmap[p] is
+// the first gauge index on cell p (-1 none), gnext chains repeated ids in
+// list order, and seed[i] is added to the Z adjoint in that order (as
+// k_gauge_seed adds it).
+constexpr int kAdjTile = 64;
+constexpr int kAdjBlock = 8 * kAdjTile;
+// one reversed cell update: flux-sum adjoints WH WU WV, old-state adjoints
+// H1 U1 V1, and the FNC and ZBC parts
+constexpr int kUb = 8;
 
 // The step's scalars in double (an fp32 mesh's are its float values).
 template <class R> __device__ __forceinline__ Scalars<double> scalars64(const Scalars<R> &S) {
@@ -183,127 +196,183 @@ template <class R> __device__ __forceinline__ auto bc_empty_like(const ExtView<R
     }
 }
 
-__device__ __forceinline__ double seeded_lz(const State<double> &l, int p, const int32_t *gmap, const int32_t *gnext,
+__device__ __forceinline__ double seeded_lz(double z, int p, const int32_t *gmap, const int32_t *gnext,
                                             const double *seed) {
-    double z = l.Z;
     if (seed)
         for (int g = gmap[p]; g >= 0; g = gnext[g]) z += seed[g];
     return z;
 }
 
+// The reversed update of cell q for the step whose input state and flux sums
+// are s_in, wsum, given the adjoint l of its output (seeds not yet added).
 template <class R>
 __device__ __forceinline__ UpdateBar cell_bar(const FusedView<R> &f, const R *fnc, const State<R> *s_in,
-                                              const R *wsum, const State<double> *lam_out, const int32_t *gmap,
+                                              const R *wsum, const State<double> &l, const int32_t *gmap,
                                               const int32_t *gnext, const double *seed, int q) {
     const int C = f.C;
     const State<R> s = s_in[q];
-    const State<double> l = lam_out[q];
-    const double lz = seeded_lz(l, q, gmap, gnext, seed);
+    const double lz = seeded_lz(l.Z, q, gmap, gnext, seed);
     return update_bar(scalars64(f.S), (double)s.H, (double)s.U, (double)s.V, (double)f.AREA[q], (double)fnc[q],
                       (double)wsum[q], (double)wsum[C + q], (double)wsum[2 * C + q], l.H, l.U, l.V, lz);
+}
+
+__device__ __forceinline__ void ub_store(double *ub, int C, int p, const UpdateBar &u) {
+    ub[p] = u.WH;
+    ub[C + p] = u.WU;
+    ub[2 * C + p] = u.WV;
+    ub[3 * C + p] = u.H1;
+    ub[4 * C + p] = u.U1;
+    ub[5 * C + p] = u.V1;
+    ub[6 * C + p] = u.FNC;
+    ub[7 * C + p] = u.ZBC;
+}
+
+__device__ __forceinline__ UpdateBar ub_load(const double *ub, int C, int p) {
+    UpdateBar u;
+    u.WH = ub[p];
+    u.WU = ub[C + p];
+    u.WV = ub[2 * C + p];
+    u.H1 = ub[3 * C + p];
+    u.U1 = ub[4 * C + p];
+    u.V1 = ub[5 * C + p];
+    u.FNC = ub[6 * C + p];
+    u.ZBC = ub[7 * C + p];
+    return u;
+}
+
+// One step's recorded input and flux sums, its gauge seeds, and where the
+// reversed updates go; every pointer is scenario 0's (blockIdx.y offsets).
+template <class R> struct AdjRec {
+    const State<R> *s_in;
+    const R *wsum;
+    const double *seed;   // null: no gauge seeds
+};
+
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_prep(FusedView<R> f, AdjRec<R> r, const State<double> *lam_out, double *ub,
+                                                  const int32_t *gmap, const int32_t *gnext, int nseed) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = f.C;
+    if (p >= C) return;
+    const size_t so = (size_t)blockIdx.y * C;
+    const R *fnc = f.FNC + so;
+    const UpdateBar u = cell_bar(f, fnc, r.s_in + so, r.wsum + 3 * so, lam_out[so + p], gmap, gnext,
+                                 r.seed ? r.seed + (size_t)blockIdx.y * nseed : nullptr, p);
+    ub_store(ub + kUb * so, C, p, u);
 }
 
 // R is the trajectory's type: the recorded states, flux sums and the mesh
 // statics are read in R and promoted; the adjoint arithmetic, the adjoint
 // states and the gradients are double.  blockIdx.y is the scenario of a
 // scenario batch (one mesh, S Manning fields and states): its state, flux
-// sums, adjoints, gradients and FNC sit blockIdx.y blocks of C (3C for the
-// sums) into the arrays, its gauge seeds nseed into `seed`.
+// sums, adjoints, gradients, reversed updates and FNC sit blockIdx.y blocks
+// of C (3C for the sums, kUb C for the updates) into the arrays, its gauge
+// seeds nseed into the seed arrays.
 template <class R>
-__global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<R> f, const uint8_t *twin,
-                                                            const State<R> *s_in, const R *wsum,
-                                                            const State<double> *lam_out, State<double> *lam_in,
-                                                            double *g_fnc, double *g_zbc, const int32_t *gmap,
-                                                            const int32_t *gnext, const double *seed, int nseed,
-                                                            int kt, int jt, int *unsupported) {
-    {
-        const size_t so = (size_t)blockIdx.y * f.C;
-        s_in += so;
-        wsum += 3 * so;
-        lam_out += so;
-        lam_in += so;
-        g_fnc += so;
-        g_zbc += so;
-        if (seed) seed += (size_t)blockIdx.y * nseed;
-    }
-    const R *fnc = f.FNC + (size_t)blockIdx.y * f.C;
-    __shared__ double wb[3][kAdjCells];
-    __shared__ double part[4][2][5][kAdjCells];
-    const int C = f.C;
-    const int l = threadIdx.x & (kAdjCells - 1);
-    const int w = threadIdx.x / kAdjCells;
-    const int j = w & 3;
-    const bool is_twin = w >= 4;
-    const int p = blockIdx.x * kAdjCells + l;
+__global__ void __launch_bounds__(kAdjBlock) k_adj_step(FusedView<R> f, const uint8_t *twin,
+                                                           const uint8_t *twin_kind, AdjRec<R> cur, AdjRec<R> prev,
+                                                           const double *ub_cur, double *ub_next,
+                                                           State<double> *lam_in, double *g_fnc, double *g_zbc,
+                                                           const int32_t *gmap, const int32_t *gnext, int nseed,
+                                                           int kt, int jt, int *unsupported) {
+    const size_t so = (size_t)blockIdx.y * f.C;
+    const State<R> *s_in = cur.s_in + so;
+    const R *fnc = f.FNC + so;
+    ub_cur += kUb * so;
+    lam_in += so;
+    g_fnc += so;
+    g_zbc += so;
+    // LDS: own and twin parts, [2][4][5][T] doubles
+    __shared__ double adj_lds[2 * 4 * 5 * kAdjTile];
+    const int C = f.C, T = kAdjTile;
+    const int t0 = blockIdx.x * T;
+    const bool twin_thread = threadIdx.x >= 4 * T;
+    const int jl = twin_thread ? threadIdx.x - 4 * T : threadIdx.x;
+    const int j = jl / T, l = jl - j * T;
+    const int p = t0 + l;
     const bool valid = p < C;
-    UpdateBar up;
-    if (w == 0 && valid) {
-        up = cell_bar(f, fnc, s_in, wsum, lam_out, gmap, gnext, seed, p);
-        wb[0][l] = up.WH;
-        wb[1][l] = up.WU;
-        wb[2][l] = up.WV;
-    }
-    const int e = j * C + p;
-    const int kind = valid ? (f.meta[e] & META_KIND_MASK) : K_PAD;
-    const int q = (kind != K_PAD) ? f.nbr[e] : -1;
-    int j2 = 0xff, kind2 = K_PAD;
-    if (is_twin && q >= 0) {
-        j2 = twin[e];
-        if (j2 <= 3) kind2 = f.meta[j2 * C + q] & META_KIND_MASK;
-    }
-    const bool twin_on = kind2 != K_PAD;
-    UpdateBar uq;
-    if (twin_on) uq = cell_bar(f, fnc, s_in, wsum, lam_out, gmap, gnext, seed, q);
+    auto part = [&](int twin_part, int jj, int comp, int ll) -> double & {
+        return adj_lds[((twin_part * 4 + jj) * 5 + comp) * T + ll];
+    };
+    // every first load is independent of the others and unconditional (a
+    // valid slot stands in for the threads without one), so they are all in
+    // flight together
+    const int e = j * C + (p < C ? p : C - 1);
+    const int mk = f.meta[e];
+    const int qn = f.nbr[e];
+    const int j2r = twin[e];
+    const int k2r = twin_kind[e];
+    const bool tail = !twin_thread && (j == 0) && valid && ub_next;
+    const int kind = valid ? (mk & META_KIND_MASK) : K_PAD;
+    const int q = (kind != K_PAD) ? qn : -1;
+    // the neighbour's slot facing this one (its twin), when it has a flux
+    const bool has_twin = q >= 0 && j2r <= 3 && k2r != K_PAD;
+    const bool in_tile = q >= t0 && q < t0 + T;
+    if (valid && !twin_thread)
+        for (int c = 0; c < 5; c++) part(1, j, c, l) = 0.0;
     __syncthreads();
-    SideBar acc;
     bool bad = false;
-    if (kind != K_PAD && (!is_twin || twin_on)) {
-        const State<R> sp = s_in[p];
-        const SideIn me{sp.H, sp.U, sp.V, sp.Z, (double)f.ZBC[p]};
-        SideIn nb{0, 0, 0, 0, 0};
-        if (q >= 0) {
-            const State<R> sq = s_in[q];
-            nb = SideIn{sq.H, sq.U, sq.V, sq.Z, (double)f.ZBC[q]};
+    const Scalars<double> S = scalars64(f.S);
+    // own thread: this slot, with this cell as the own side; twin thread (a
+    // neighbour outside the workgroup's cells): the neighbour's facing slot,
+    // with the neighbour as the own side
+    const bool work = twin_thread ? (kind != K_PAD && has_twin && !in_tile) : kind != K_PAD;
+    if (work) {
+        {
+            const bool tw = twin_thread;
+            const State<R> sp = s_in[p];
+            const SideIn me{sp.H, sp.U, sp.V, sp.Z, (double)f.ZBC[p]};
+            SideIn nb{0, 0, 0, 0, 0};
+            if (q >= 0) {
+                const State<R> sq = s_in[q];
+                nb = SideIn{sq.H, sq.U, sq.V, sq.Z, (double)f.ZBC[q]};
+            }
+            const int es = tw ? j2r * C + q : e;
+            const int ks = tw ? k2r : kind;
+            const int oc = tw ? q : p, js = tw ? j2r : j;
+            UpdateBar ws;
+            ws.WH = ub_cur[oc];
+            ws.WU = ub_cur[C + oc];
+            ws.WV = ub_cur[2 * C + oc];
+            // SIDE*COSF and SIDE*SINF in the trajectory's type, as the
+            // forward step forms them
+            const R cR = f.COSF[es], snR = f.SINF[es], SLR = f.SIDE[es];
+            const double c = cR, sn = snR, SL = SLR;
+            const double SLCA = f.SLCOS ? (double)f.SLCOS[es] : (double)(R)(SLR * cR);
+            const double SLSA = f.SLSIN ? (double)f.SLSIN[es] : (double)(R)(SLR * snR);
+            double Fb[4];
+            flux_bar_of_sums(ws, SL, SLCA, SLSA, Fb);
+            auto bc = bc_empty_like(*f.ext);
+            if (ks >= K_B10) bc = bc_of(*f.ext, f.orig[oc], 4 * f.orig[oc] + js, kt, jt);
+            // one call for both passes: the own side is the neighbour on the twin pass
+            const SideIn A = tw ? nb : me, B = tw ? me : nb;
+            SideBar a_own, a_nb;
+            bad |= !slot_flux_bar(ks, S, bc, A, tw || q >= 0, B, c, sn, Fb, a_own, a_nb);
+            // pass 0: own part here, the neighbour's part to its twin entry
+            // when the neighbour is in the tile; pass 1: this cell's twin part
+            const SideBar &mine = tw ? a_nb : a_own;
+            const int which = tw ? 1 : 0;
+            part(which, j, 0, l) = mine.H;
+            part(which, j, 1, l) = mine.U;
+            part(which, j, 2, l) = mine.V;
+            part(which, j, 3, l) = mine.Z;
+            part(which, j, 4, l) = mine.ZB;
+            if (!tw && has_twin && in_tile) {
+                const int lq = q - t0;
+                part(1, j2r, 0, lq) = a_nb.H;
+                part(1, j2r, 1, lq) = a_nb.U;
+                part(1, j2r, 2, lq) = a_nb.V;
+                part(1, j2r, 3, lq) = a_nb.Z;
+                part(1, j2r, 4, lq) = a_nb.ZB;
+            }
         }
-        // the slot whose flux is reversed: this cell's slot j, or the
-        // neighbour's slot j2 with the roles of the two states swapped
-        const int es = is_twin ? j2 * C + q : e;
-        const int ks = is_twin ? kind2 : kind;
-        UpdateBar ws;
-        if (is_twin) {
-            ws = uq;
-        } else {
-            ws.WH = wb[0][l];
-            ws.WU = wb[1][l];
-            ws.WV = wb[2][l];
-        }
-        // SIDE*COSF and SIDE*SINF in the trajectory's type, as the forward
-        // step forms them
-        const R cR = f.COSF[es], snR = f.SINF[es], SLR = f.SIDE[es];
-        const double c = cR, sn = snR, SL = SLR;
-        const double SLCA = f.SLCOS ? (double)f.SLCOS[es] : (double)(R)(SLR * cR);
-        const double SLSA = f.SLSIN ? (double)f.SLSIN[es] : (double)(R)(SLR * snR);
-        double Fb[4];
-        flux_bar_of_sums(ws, SL, SLCA, SLSA, Fb);
-        const int oc = is_twin ? q : p, js = is_twin ? j2 : j;
-        auto bc = bc_empty_like(*f.ext);
-        if (ks >= K_B10) bc = bc_of(*f.ext, f.orig[oc], 4 * f.orig[oc] + js, kt, jt);
-        const Scalars<double> S = scalars64(f.S);
-        SideBar other;
-        if (is_twin)
-            bad |= !slot_flux_bar(ks, S, bc, nb, true, me, c, sn, Fb, other, acc);
-        else
-            bad |= !slot_flux_bar(ks, S, bc, me, q >= 0, nb, c, sn, Fb, acc, other);
+    } else if (valid && !twin_thread) {
+        for (int c = 0; c < 5; c++) part(0, j, c, l) = 0.0;
     }
-    const int k = is_twin ? 1 : 0;
-    part[j][k][0][l] = acc.H;
-    part[j][k][1][l] = acc.U;
-    part[j][k][2][l] = acc.V;
-    part[j][k][3][l] = acc.Z;
-    part[j][k][4][l] = acc.ZB;
     if (bad) *unsupported = 1;
     __syncthreads();
-    if (w != 0 || !valid) return;
+    if (twin_thread || j != 0 || !valid) return;
+    const UpdateBar up = ub_load(ub_cur, C, p);
     State<double> lam{up.H1, up.U1, up.V1, 0.0};
     double gz = 0;
     for (int jj = 0; jj < 4; jj++) {
@@ -311,16 +380,21 @@ __global__ void __launch_bounds__(8 * kAdjCells) k_adj_step(FusedView<R> f, cons
             // a padding slot or a missing twin contributes exact zeros, which
             // k_adj_flux skips; adding +0.0 leaves every sum's bits unchanged
             // except -0.0 + 0.0, and lam starts from the update's adjoints
-            lam.H += part[jj][kk][0][l];
-            lam.U += part[jj][kk][1][l];
-            lam.V += part[jj][kk][2][l];
-            lam.Z += part[jj][kk][3][l];
-            gz += part[jj][kk][4][l];
+            lam.H += part(kk, jj, 0, l);
+            lam.U += part(kk, jj, 1, l);
+            lam.V += part(kk, jj, 2, l);
+            lam.Z += part(kk, jj, 3, l);
+            gz += part(kk, jj, 4, l);
         }
     }
     lam_in[p] = lam;
     g_fnc[p] += up.FNC;
     g_zbc[p] = (g_zbc[p] + up.ZBC) + gz;
+    if (tail) {   // the previous step's update, reversed from the adjoint just formed
+        const UpdateBar u = cell_bar(f, fnc, prev.s_in + so, prev.wsum + 3 * so, lam, gmap, gnext,
+                                     prev.seed ? prev.seed + (size_t)blockIdx.y * nseed : nullptr, p);
+        ub_store(ub_next + kUb * so, C, p, u);
+    }
 }
 
 // Gauge tables of the fused adjoint step: gmap[c] = first index i of the
@@ -484,24 +558,42 @@ hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, 
 }
 
 template <class R>
-hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const State<R> *s_in, const R *wsum,
-                                     const State<double> *lam_out, State<double> *lam_in, double *g_fnc,
-                                     double *g_zbc, const int32_t *gmap, const int32_t *gnext, const double *seed,
-                                     int nseed, int nscen, int kt, int jt, int *unsupported, hipStream_t s) {
+hipError_t launch_adjoint_prep(const FusedView<R> &f, const State<R> *s_in, const R *wsum, const double *seed,
+                               const State<double> *lam_out, double *ub, const int32_t *gmap, const int32_t *gnext,
+                               int nseed, int nscen, hipStream_t s) {
     if (f.C == 0 || nscen <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_adj_step<R>), dim3(div_up(f.C, kAdjCells), nscen), dim3(8 * kAdjCells), 0, s, f, twin,
-                       s_in, wsum, lam_out, lam_in, g_fnc, g_zbc, gmap, gnext, seed, nseed, kt, jt, unsupported);
+    const AdjRec<R> r{s_in, wsum, seed};
+    hipLaunchKernelGGL((k_adj_prep<R>), dim3(div_up(f.C, 256), nscen), dim3(256), 0, s, f, r, lam_out, ub, gmap, gnext,
+                       nseed);
     return hipGetLastError();
 }
-template hipError_t launch_adjoint_step_fused<double>(const FusedView<double> &, const uint8_t *,
-                                                      const State<double> *, const double *, const State<double> *,
-                                                      State<double> *, double *, double *, const int32_t *,
-                                                      const int32_t *, const double *, int, int, int, int, int *,
-                                                      hipStream_t);
-template hipError_t launch_adjoint_step_fused<float>(const FusedView<float> &, const uint8_t *, const State<float> *,
-                                                     const float *, const State<double> *, State<double> *, double *,
-                                                     double *, const int32_t *, const int32_t *, const double *, int,
-                                                     int, int, int, int *, hipStream_t);
+
+template <class R>
+hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const uint8_t *twin_kind,
+                                     const State<R> *s_in, const R *wsum,
+                                     const State<R> *s_prev, const R *wsum_prev, const double *seed_prev,
+                                     const double *ub_cur, double *ub_next, State<double> *lam_in, double *g_fnc,
+                                     double *g_zbc, const int32_t *gmap, const int32_t *gnext, int nseed, int nscen,
+                                     int kt, int jt, int *unsupported, hipStream_t s) {
+    if (f.C == 0 || nscen <= 0) return hipSuccess;
+    const AdjRec<R> cur{s_in, wsum, nullptr}, prev{s_prev, wsum_prev, seed_prev};
+    hipLaunchKernelGGL((k_adj_step<R>), dim3(div_up(f.C, kAdjTile), nscen), dim3(kAdjBlock), 0, s, f, twin, twin_kind,
+                       cur, prev, ub_cur, ub_next, lam_in, g_fnc, g_zbc, gmap, gnext, nseed, kt, jt, unsupported);
+    return hipGetLastError();
+}
+
+#define HYDRO_ADJ_INST(R)                                                                                        \
+    template hipError_t launch_adjoint_prep<R>(const FusedView<R> &, const State<R> *, const R *, const double *, \
+                                               const State<double> *, double *, const int32_t *, const int32_t *, \
+                                               int, int, hipStream_t);                                          \
+    template hipError_t launch_adjoint_step_fused<R>(const FusedView<R> &, const uint8_t *, const uint8_t *,      \
+                                                     const State<R> *,                                           \
+                                                     const R *, const State<R> *, const R *, const double *,      \
+                                                     const double *, double *, State<double> *, double *, double *, \
+                                                     const int32_t *, const int32_t *, int, int, int, int, int *, \
+                                                     hipStream_t);
+HYDRO_ADJ_INST(double)
+HYDRO_ADJ_INST(float)
 
 hipError_t launch_gauge_map(const int32_t *inv, const int32_t *cells, int n, int C, int32_t *gmap, int32_t *gnext,
                             int32_t *internal, hipStream_t s) {

@@ -109,6 +109,7 @@ struct hydro_mesh {
     int cur = 0;  // state record buffer holding the current state
     bool ext_valid = true, int_valid = false;
     DevBuf st[2], ZBC, AREA, FNC, DTA, ZB1, nbr, meta, COSF, SINF, SIDE, SLCOS, SLSIN, orig, inv, twin;
+    DevBuf twin_kind;   // [4][C]: kind of each slot's twin slot (K_PAD none), read by the fused adjoint step
     DevBuf task_w, task_nb, task_cs, task_sd, task_n, cell_pad;   // TILES scheme task entries
     DevBuf tile_nb_off, tile_nb, tile_flag;   // persistent TILES kernel: neighbour tiles (CSR), step flags
     int max_tile_nb = 0;
@@ -117,6 +118,11 @@ struct hydro_mesh {
     int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
+    DevBuf adj_ub;           // misfit sweeps: reversed cell updates, [2][S][kUbDoubles][C] doubles
+    // misfit sweeps: the side stream the recomputation runs on, and the
+    // events ordering it with the adjoint steps (created on first use)
+    hipStream_t side_stream = nullptr;
+    hipEvent_t sweep_start = nullptr, rec_done[2] = {}, adj_done[2] = {};
     DevBuf scen_st[2], scen_fnc;   // scenario misfit loops: states [S][C] (ping-pong), FNC [S][C]
     DevBuf gmap, gnext, gcells;   // misfit loops: gauge map [C], chain [n], internal ids [n]
     DevBuf tick, barrier, perr, ext_dev;   // tick: StepClock of the GRAPH mode
@@ -139,6 +145,9 @@ struct hydro_mesh {
     ~hydro_mesh() {
         graphs.clear();
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        if (side_stream) (void)hipStreamDestroy(side_stream);
+        for (hipEvent_t e : {sweep_start, rec_done[0], rec_done[1], adj_done[0], adj_done[1]})
+            if (e) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
     }
@@ -211,7 +220,8 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.tile_nb_off = m->tile_nb_off.as<int32_t>();
     f.tile_nb = m->tile_nb.as<int32_t>();
     f.tile_flag = m->tile_flag.as<int32_t>();
-    f.gauge_cells = nullptr;
+    f.gauge_map = nullptr;
+    f.gauge_next = nullptr;
     f.gauge_n = 0;
     f.gauge_out = nullptr;
     return f;
@@ -508,6 +518,18 @@ template <class R> int prepare_t(hydro_mesh *m) {
         (rc = h2d(m->SLCOS, slc_n)) || (rc = h2d(m->SLSIN, sls_n)) || (rc = h2d(m->orig, orig_n)) ||
         (rc = h2d(m->inv, inv)) || (rc = h2d(m->twin, twin_n)))
         return rc;
+    {
+        // the kind of each slot's twin (the neighbour's slot facing it), for
+        // the fused adjoint step's first loads; K_PAD where there is none
+        std::vector<uint8_t> tk(S4, (uint8_t)K_PAD);
+        for (int j = 0; j < 4; j++)
+            for (int p = 0; p < C; p++) {
+                const size_t sn = (size_t)j * C + p;
+                const int q = nbr_n[sn];
+                if (q >= 0 && twin_n[sn] <= 3) tk[sn] = meta_n[(size_t)twin_n[sn] * C + q] & META_KIND_MASK;
+            }
+        if ((rc = h2d(m->twin_kind, tk))) return rc;
+    }
     {
         const ExtView<R> ev = ext_view<R>(m);
         if ((rc = m->ext_dev.alloc(sizeof(ExtView<R>)))) return rc;
@@ -1268,12 +1290,17 @@ int hydro_state_load(hydro_mesh *m, const void *state, hipStream_t s) {
     return HYDRO_OK;
 }
 
-extern "C++" template <class R, int VAR> static int record_t(hydro_mesh *m, int n, void *states, void *wsums, hipStream_t s) {
+// first_in (optional): the first step reads its input from there (a
+// checkpoint) instead of the current state buffer.
+extern "C++" template <class R, int VAR>
+static int record_t(hydro_mesh *m, int n, void *states, void *wsums, hipStream_t s,
+                    const State<R> *first_in = nullptr) {
     int rc = ensure_internal<R>(m, s);
     if (rc) return rc;
     const size_t C = (size_t)m->CELL;
     for (int i = 0; i < n; i++) {
         FusedView<R> f = fused_view<R>(m, m->cur);
+        if (i == 0 && first_in) f.Sin = first_in;
         f.rec_state = (State<R> *)states + (size_t)i * C;
         f.rec_wsum = (R *)wsums + (size_t)i * 3 * C;
         HIP_TRY((launch_fused_step<R, VAR>(f, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
@@ -1383,12 +1410,11 @@ static int misfit_forward_t(hydro_mesh *m, int nsteps, int seg, void *ckpt, cons
     if ((rc = ensure_internal<R>(m, s)) || (rc = gauge_tables(m, gauges, n, s))) return rc;
     const size_t C = (size_t)m->CELL;
     for (int k = 0; k < nsteps; k++) {
-        if (ckpt && k % seg == 0)
-            HIP_TRY(hipMemcpyAsync((State<R> *)ckpt + (size_t)(k / seg) * C, m->st[m->cur].p, m->st[m->cur].bytes,
-                                   hipMemcpyDeviceToDevice, s));
         FusedView<R> f = fused_view<R>(m, m->cur);
+        if (ckpt && k % seg == 0) f.rec_state = (State<R> *)ckpt + (size_t)(k / seg) * C;   // written by the step
         if (n > 0 && k > 0) {   // this step's input is the previous step's output
-            f.gauge_cells = m->gcells.as<int32_t>();
+            f.gauge_map = m->gmap.as<int32_t>();
+            f.gauge_next = m->gnext.as<int32_t>();
             f.gauge_n = n;
             f.gauge_out = zg + (size_t)(k - 1) * n;
         }
@@ -1415,6 +1441,76 @@ int hydro_misfit_forward(hydro_mesh *m, int nsteps, int seg, void *ckpt, const i
                                    : misfit_forward_t<float>(m, nsteps, seg, ckpt, gauges, n, zg, s);
 }
 
+// The adjoint steps of one recorded segment (steps s0 .. s0+len-1, records
+// i = 0 .. len-1, S scenarios per record), last step first: the last step's
+// cell updates are reversed from L[cur] first, then every adjoint step also
+// reverses the update of the step before it.  On return L[cur] holds the
+// adjoint of the segment's input state.
+extern "C++" template <class R>
+static int adjoint_segment(hydro_mesh *m, const FusedView<R> &f, int S, int s0, int len, int64_t g0, int n,
+                           const double *resid, void *states, void *wsums, State<double> *L[2], int &cur,
+                           double *g_fnc, double *g_zbc, hipStream_t s) {
+    const int per = m->spd - 1;
+    const size_t rec = (size_t)S * m->CELL;
+    const size_t ub_rec = (size_t)kUbDoubles * rec;
+    int rc;
+    if (m->adj_ub.bytes < 2 * ub_rec * sizeof(double) && (rc = m->adj_ub.alloc(2 * ub_rec * sizeof(double))))
+        return rc;
+    double *ub[2] = {m->adj_ub.as<double>(), m->adj_ub.as<double>() + ub_rec};
+    auto st = [&](int i) { return (const State<R> *)states + (size_t)i * rec; };
+    auto ws = [&](int i) { return (const R *)wsums + (size_t)i * 3 * rec; };
+    auto seed = [&](int i) { return n > 0 ? resid + (size_t)(s0 + i) * S * n : nullptr; };
+    const int32_t *gmap = m->gmap.as<int32_t>(), *gnext = m->gnext.as<int32_t>();
+    int u = 0;
+    HIP_TRY(launch_adjoint_prep<R>(f, st(len - 1), ws(len - 1), seed(len - 1), L[cur], ub[u], gmap, gnext, n, S, s));
+    for (int i = len - 1; i >= 0; i--) {
+        const int64_t g = g0 + s0 + i;
+        const bool prev = i > 0;
+        HIP_TRY(launch_adjoint_step_fused<R>(f, m->twin.as<uint8_t>(), m->twin_kind.as<uint8_t>(), st(i), ws(i),
+                                             prev ? st(i - 1) : nullptr,
+                                             prev ? ws(i - 1) : nullptr, prev ? seed(i - 1) : nullptr, ub[u],
+                                             prev ? ub[u ^ 1] : nullptr, L[cur ^ 1], g_fnc, g_zbc, gmap, gnext, n, S,
+                                             1 + (int)(g % per), (int)(g / per), m->adj_flag.as<int>(), s));
+        cur ^= 1;
+        u ^= 1;
+    }
+    return HYDRO_OK;
+}
+
+// The adjoint sweep of nseg segments, last first, with the recomputation of
+// segment k-1 (rec, on the mesh's side stream) overlapping the adjoint steps
+// of segment k (adj, on the caller's stream): rec(k, b, stream) records
+// segment k into record buffer b = k & 1, adj(k, b, stream) runs its adjoint
+// steps from it.  Events order each adjoint after its segment's recording and
+// each recording after the adjoint that last read its buffer; the caller's
+// stream ends after all of it.  Both streams have the default priority.
+extern "C++" template <class RecFn, class AdjFn>
+static int pipelined_sweep(hydro_mesh *m, int nseg, hipStream_t s, RecFn rec, AdjFn adj) {
+    if (nseg <= 0) return HYDRO_OK;
+    if (!m->side_stream) HIP_TRY(hipStreamCreateWithFlags(&m->side_stream, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&m->sweep_start, &m->rec_done[0], &m->rec_done[1], &m->adj_done[0], &m->adj_done[1]})
+        if (!*e) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    hipStream_t side = m->side_stream;
+    HIP_TRY(hipEventRecord(m->sweep_start, s));   // everything the caller issued before
+    HIP_TRY(hipStreamWaitEvent(side, m->sweep_start, 0));
+    int rc;
+    if ((rc = rec(nseg - 1, (nseg - 1) & 1, side))) return rc;
+    HIP_TRY(hipEventRecord(m->rec_done[(nseg - 1) & 1], side));
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int b = k & 1;
+        if (k >= 1) {
+            const int bb = (k - 1) & 1;
+            if (k + 1 <= nseg - 1) HIP_TRY(hipStreamWaitEvent(side, m->adj_done[bb], 0));   // adj(k+1) read bb
+            if ((rc = rec(k - 1, bb, side))) return rc;
+            HIP_TRY(hipEventRecord(m->rec_done[bb], side));
+        }
+        HIP_TRY(hipStreamWaitEvent(s, m->rec_done[b], 0));
+        if ((rc = adj(k, b, s))) return rc;
+        HIP_TRY(hipEventRecord(m->adj_done[b], s));
+    }
+    return HYDRO_OK;
+}
+
 extern "C++" template <class R>
 static int misfit_backward_t(hydro_mesh *m, int nsteps, int seg, const void *ckpt, int n, const double *resid,
                              void *states, void *wsums, void *lam, double *g_fnc, double *g_zbc, int *lam_index,
@@ -1424,29 +1520,26 @@ static int misfit_backward_t(hydro_mesh *m, int nsteps, int seg, const void *ckp
     const size_t C = (size_t)m->CELL;
     State<double> *L[2] = {(State<double> *)lam, (State<double> *)lam + C};
     HIP_TRY(hipMemsetAsync(L[0], 0, C * sizeof(State<double>), s));
-    int rc, cur = 0;
+    int cur = 0;
     const int nseg = (nsteps + seg - 1) / seg;
-    for (int k = nseg - 1; k >= 0; k--) {
-        const int s0 = k * seg;
-        const int len = std::min(seg, nsteps - s0);
-        HIP_TRY(hipMemcpyAsync(m->st[m->cur].p, (const State<R> *)ckpt + (size_t)k * C, m->st[m->cur].bytes,
-                               hipMemcpyDeviceToDevice, s));
+    auto st_b = [&](int b) { return (void *)((State<R> *)states + (size_t)b * seg * C); };
+    auto ws_b = [&](int b) { return (void *)((R *)wsums + (size_t)b * seg * 3 * C); };
+    auto rec = [&](int k, int b, hipStream_t st) {
+        const int s0 = k * seg, len = std::min(seg, nsteps - s0);
+        // the segment again from its checkpoint (read in place by the first step)
         m->int_valid = true;
         m->ext_valid = false;
         m->day = (int)((g0 + s0) / per);
         m->kt = 1 + (int)((g0 + s0) % per);
-        if ((rc = record_t<R, 0>(m, len, states, wsums, s))) return rc;
-        const FusedView<R> f = fused_view<R>(m, m->cur);
-        for (int i = len - 1; i >= 0; i--) {
-            const int64_t g = g0 + s0 + i;
-            HIP_TRY(launch_adjoint_step_fused<R>(f, m->twin.as<uint8_t>(), (const State<R> *)states + (size_t)i * C,
-                                                 (const R *)wsums + (size_t)i * 3 * C, L[cur], L[cur ^ 1], g_fnc,
-                                                 g_zbc, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
-                                                 n > 0 ? resid + (size_t)(s0 + i) * n : nullptr, n, 1,
-                                                 1 + (int)(g % per), (int)(g / per), m->adj_flag.as<int>(), s));
-            cur ^= 1;
-        }
-    }
+        return record_t<R, 0>(m, len, st_b(b), ws_b(b), st, (const State<R> *)ckpt + (size_t)k * C);
+    };
+    auto adj = [&](int k, int b, hipStream_t st) {
+        const int s0 = k * seg, len = std::min(seg, nsteps - s0);
+        return adjoint_segment<R>(m, fused_view<R>(m, m->cur), 1, s0, len, g0, n, resid, st_b(b), ws_b(b), L, cur,
+                                  g_fnc, g_zbc, st);
+    };
+    int rc = pipelined_sweep(m, nseg, s, rec, adj);
+    if (rc) return rc;
     *lam_index = cur;
     return HYDRO_OK;
 }
@@ -1521,12 +1614,11 @@ static int scen_forward_t(hydro_mesh *m, int S, const double *fnc, int nsteps, i
     const int day0 = m->day, kt0 = m->kt;
     int cur = 0;
     for (int k = 0; k < nsteps; k++) {
-        if (ckpt && k % seg == 0)
-            HIP_TRY(hipMemcpyAsync((State<R> *)ckpt + (size_t)(k / seg) * rec, m->scen_st[cur].p, rec * sizeof(State<R>),
-                                   hipMemcpyDeviceToDevice, s));
         FusedView<R> f = scen_view<R>(m, cur);
+        if (ckpt && k % seg == 0) f.rec_state = (State<R> *)ckpt + (size_t)(k / seg) * rec;   // written by the step
         if (n > 0 && k > 0) {
-            f.gauge_cells = m->gcells.as<int32_t>();
+            f.gauge_map = m->gmap.as<int32_t>();
+            f.gauge_next = m->gnext.as<int32_t>();
             f.gauge_n = n;
             f.gauge_out = zg + (size_t)(k - 1) * S * n;
         }
@@ -1556,33 +1648,30 @@ static int scen_backward_t(hydro_mesh *m, int S, const double *fnc, int nsteps, 
     HIP_TRY(hipMemsetAsync(L[0], 0, rec * sizeof(State<double>), s));
     int cur = 0;
     const int nseg = (nsteps + seg - 1) / seg;
-    for (int k = nseg - 1; k >= 0; k--) {
-        const int s0 = k * seg;
-        const int len = std::min(seg, nsteps - s0);
-        HIP_TRY(hipMemcpyAsync(m->scen_st[0].p, (const State<R> *)ckpt + (size_t)k * rec, rec * sizeof(State<R>),
-                               hipMemcpyDeviceToDevice, s));
+    auto st_b = [&](int b) { return (State<R> *)states + (size_t)b * seg * rec; };
+    auto ws_b = [&](int b) { return (R *)wsums + (size_t)b * seg * 3 * rec; };
+    auto rec_fn = [&](int k, int b, hipStream_t st) -> int {
+        const int s0 = k * seg, len = std::min(seg, nsteps - s0);
         m->day = (int)((g0 + s0) / per);
         m->kt = 1 + (int)((g0 + s0) % per);
         int sc = 0;
         for (int i = 0; i < len; i++) {   // the segment again, recording each step's input and flux sums
             FusedView<R> f = scen_view<R>(m, sc);
-            f.rec_state = (State<R> *)states + (size_t)i * rec;
-            f.rec_wsum = (R *)wsums + (size_t)i * 3 * rec;
-            HIP_TRY((launch_fused_step_scen<R, 0>(f, S, m->kt, m->day, m->tune.block, m->tune.xcd_remap, s)));
+            if (i == 0) f.Sin = (const State<R> *)ckpt + (size_t)k * rec;   // the checkpoint, read in place
+            f.rec_state = st_b(b) + (size_t)i * rec;
+            f.rec_wsum = ws_b(b) + (size_t)i * 3 * rec;
+            HIP_TRY((launch_fused_step_scen<R, 0>(f, S, m->kt, m->day, m->tune.block, m->tune.xcd_remap, st)));
             sc ^= 1;
             advance(m);
         }
-        const FusedView<R> f = scen_view<R>(m, 0);
-        for (int i = len - 1; i >= 0; i--) {
-            const int64_t g = g0 + s0 + i;
-            HIP_TRY(launch_adjoint_step_fused<R>(f, m->twin.as<uint8_t>(), (const State<R> *)states + (size_t)i * rec,
-                                                 (const R *)wsums + (size_t)i * 3 * rec, L[cur], L[cur ^ 1], g_fnc,
-                                                 g_zbc, m->gmap.as<int32_t>(), m->gnext.as<int32_t>(),
-                                                 n > 0 ? resid + (size_t)(s0 + i) * S * n : nullptr, n, S,
-                                                 1 + (int)(g % per), (int)(g / per), m->adj_flag.as<int>(), s));
-            cur ^= 1;
-        }
-    }
+        return HYDRO_OK;
+    };
+    auto adj_fn = [&](int k, int b, hipStream_t st) {
+        const int s0 = k * seg, len = std::min(seg, nsteps - s0);
+        return adjoint_segment<R>(m, scen_view<R>(m, 0), S, s0, len, g0, n, resid, st_b(b), ws_b(b), L, cur, g_fnc,
+                                  g_zbc, st);
+    };
+    if ((rc = pipelined_sweep(m, nseg, s, rec_fn, adj_fn))) return rc;
     m->day = day0;
     m->kt = kt0;
     *lam_index = cur;

@@ -47,6 +47,13 @@ __global__ void __launch_bounds__(kMaxBlock) k_update_cell(ExtView<R> m) {
 // ---------------------------------------------------------------------------
 // Fused step on the reordered layout.
 
+// Gauge levels of a step's input state: cell p's value to every gauge on p.
+template <class R>
+__device__ __forceinline__ void gauge_write(const FusedView<R> &f, int p, R z, int scen) {
+    double *out = f.gauge_out + (size_t)scen * f.gauge_n;
+    for (int g = f.gauge_map[p]; g >= 0; g = f.gauge_next[g]) out[g] = (double)z;
+}
+
 // RARE = false: the mesh has only interior, KLAS 10/1/4 and padding slots
 // (checked on the host), so the other boundary handlers are not compiled in.
 template <class R, int VAR, bool RARE>
@@ -116,12 +123,13 @@ __device__ __forceinline__ void fused_cell(const FusedView<R> &f, const State<R>
             m.F0[eo] = fl.f0; m.F1[eo] = fl.f1; m.F2[eo] = fl.f2; m.F3[eo] = fl.f3;
         }
     }
+    if (f.rec_state) f.rec_state[p] = me;
     if (f.rec_wsum) {
-        f.rec_state[p] = me;
         f.rec_wsum[p] = WH;
         f.rec_wsum[C + p] = WU;
         f.rec_wsum[2 * C + p] = WV;
     }
+    if (f.gauge_out) gauge_write(f, p, me.Z, 0);
     const CellOut<R> o = update_math<R, VAR>(S, me.H, me.U, me.V, ZBp, f.AREA[p], f.FNC[p], WH, WU, WV,
                                              sd[0], sd[1], sd[2], sd[3]);
     Sout[p] = State<R>{o.H, o.U, o.V, o.Z};   // W = speed(U, V) is formed on the way out
@@ -429,9 +437,10 @@ __device__ __forceinline__ bool tile_cells(const FusedView<R> &f, const State<R>
             WV += L.c[(8 + j) * B + i];
         }
     }
+    if (f.rec_state) f.rec_state[(SCEN ? (size_t)scen * C : 0) + p] = me;
+    if (f.gauge_out) gauge_write(f, p, me.Z, scen);
     if (f.rec_wsum) {
         const size_t so = SCEN ? (size_t)scen * C : 0;
-        f.rec_state[so + p] = me;
         f.rec_wsum[3 * so + p] = WH;
         f.rec_wsum[3 * so + C + p] = WU;
         f.rec_wsum[3 * so + 2 * C + p] = WV;
@@ -455,15 +464,8 @@ __device__ __forceinline__ void step_tile(const FusedView<R> &f, const State<R> 
     }
 }
 
-// Gauge levels of the step's input state (the misfit loops' per-step read).
-template <class R> __device__ __forceinline__ void gauge_read(const FusedView<R> &f) {
-    if (f.gauge_out != nullptr && blockIdx.x == 0)
-        for (int i = threadIdx.x; i < f.gauge_n; i += blockDim.x) f.gauge_out[i] = (double)f.Sin[f.gauge_cells[i]].Z;
-}
-
 template <class R, int VAR, bool RARE, int SCHEME>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt, int jt, int remap) {
-    gauge_read(f);
     step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
 
@@ -474,9 +476,6 @@ template <class R, int VAR, bool RARE>
 __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_scen(FusedView<R> f, int kt, int jt, int remap) {
     const int scen = blockIdx.x / f.ntiles, wg = blockIdx.x - scen * f.ntiles;
     const size_t so = (size_t)scen * f.C;
-    if (f.gauge_out != nullptr && wg == 0)
-        for (int i = threadIdx.x; i < f.gauge_n; i += blockDim.x)
-            f.gauge_out[(size_t)scen * f.gauge_n + i] = (double)f.Sin[so + f.gauge_cells[i]].Z;
     tile_cells<R, VAR, RARE, false, true>(f, f.Sin + so, f.Sout + so, tile_of(wg, f.ntiles, remap), kt, jt, scen);
 }
 
@@ -485,7 +484,6 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_scen(FusedView<R>
 template <class R, int VAR, bool RARE>
 __global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_fused_step_o4(FusedView<R> f, int kt, int jt, int remap) {
-    gauge_read(f);
     tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
 

@@ -70,8 +70,8 @@ template <class R> struct FusedView {
     const R *task_cs, *task_sd;
     const int32_t *task_n;
     const uint8_t *cell_pad;
-    // recording for the adjoint (null when not recording): the step's input
-    // state, and its flux sums WH, WU, WV as [3][C]
+    // recording (each null when off): the step's input state (for the
+    // adjoint, or as a checkpoint), and its flux sums WH, WU, WV as [3][C]
     State<R> *rec_state;
     R *rec_wsum;
     // persistent TILES kernel: tiles sharing an edge with tile t are
@@ -79,9 +79,11 @@ template <class R> struct FusedView {
     // the launch that tile t has completed
     const int32_t *tile_nb_off, *tile_nb;
     int32_t *tile_flag;
-    // gauge read (null when off): workgroup 0 copies Z of the step's INPUT
-    // state at the internal cells gauge_cells[0..gauge_n) to gauge_out
-    const int32_t *gauge_cells;
+    // gauge read (gauge_out null when off): the thread of internal cell p
+    // writes Z of the step's INPUT state to gauge_out[g] for every gauge g on
+    // p (gauge_map[p] the first, -1 none; gauge_next[g] the next gauge on the
+    // same cell, -1 none); scenario s writes at gauge_out + s * gauge_n
+    const int32_t *gauge_map, *gauge_next;
     int gauge_n;
     double *gauge_out;
 };
@@ -190,11 +192,23 @@ hipError_t launch_adjoint_step(const FusedView<double> &f, const uint8_t *twin, 
 hipError_t launch_tlm_step(const FusedView<double> &f, const State<double> *s_in, const State<double> *ds_in,
                            const double *dfnc, const double *dzbc, State<double> *s_out, State<double> *ds_out, int kt,
                            int jt, hipStream_t s);
+// The fused adjoint step reads every cell's reversed update of its step from
+// ub_cur (kUbDoubles doubles per cell: [8][C] per scenario) and, when
+// ub_next is set, reverses the previous step's update (input s_prev, sums
+// wsum_prev, gauge seeds seed_prev) into ub_next from the adjoint it forms;
+// launch_adjoint_prep fills ub for a segment's last step from lam_out.
+constexpr int kUbDoubles = 8;
 template <class R>
-hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const State<R> *s_in, const R *wsum,
-                                     const State<double> *lam_out, State<double> *lam_in, double *g_fnc,
-                                     double *g_zbc, const int32_t *gmap, const int32_t *gnext, const double *seed,
-                                     int nseed, int nscen, int kt, int jt, int *unsupported, hipStream_t s);
+hipError_t launch_adjoint_prep(const FusedView<R> &f, const State<R> *s_in, const R *wsum, const double *seed,
+                               const State<double> *lam_out, double *ub, const int32_t *gmap, const int32_t *gnext,
+                               int nseed, int nscen, hipStream_t s);
+template <class R>
+hipError_t launch_adjoint_step_fused(const FusedView<R> &f, const uint8_t *twin, const uint8_t *twin_kind,
+                                     const State<R> *s_in, const R *wsum,
+                                     const State<R> *s_prev, const R *wsum_prev, const double *seed_prev,
+                                     const double *ub_cur, double *ub_next, State<double> *lam_in, double *g_fnc,
+                                     double *g_zbc, const int32_t *gmap, const int32_t *gnext, int nseed, int nscen,
+                                     int kt, int jt, int *unsupported, hipStream_t s);
 template <class R>
 hipError_t launch_gauge_gather_scen(const State<R> *st, int C, int nscen, const int32_t *inv, const int32_t *cells,
                                     int n, double *out, hipStream_t s);

@@ -85,8 +85,10 @@ class _Buffers:
         # mesh's type; adjoint states and gradients in double
         tt = torch.float64 if dm.host.dtype == np.float64 else torch.float32
         self.ckpt = torch.empty((nseg + 1, C, 4), dtype=tt, device=dev)
-        self.states = torch.empty((seg + 1, C, 4), dtype=tt, device=dev)
-        self.wsums = torch.empty((seg, 3, C), dtype=tt, device=dev)
+        # two segments of records: one is recomputed while the other's
+        # adjoint steps run
+        self.states = torch.empty((2 * seg, C, 4), dtype=tt, device=dev)
+        self.wsums = torch.empty((2 * seg, 3, C), dtype=tt, device=dev)
         self.lam = torch.zeros((2, C, 4), dtype=torch.float64, device=dev)
         self.g_fnc = torch.zeros(C, dtype=torch.float64, device=dev)
         self.g_zbc = torch.zeros(C, dtype=torch.float64, device=dev)
@@ -186,8 +188,8 @@ class _ScenBuffers:
         tt = torch.float64 if dm.host.dtype == np.float64 else torch.float32
         self.S, self.C = S, C
         self.ckpt = torch.empty((nseg + 1, S, C, 4), dtype=tt, device=dev)
-        self.states = torch.empty((seg + 1, S, C, 4), dtype=tt, device=dev)
-        self.wsums = torch.empty((seg, S, 3, C), dtype=tt, device=dev)
+        self.states = torch.empty((2 * seg, S, C, 4), dtype=tt, device=dev)
+        self.wsums = torch.empty((2 * seg, S, 3, C), dtype=tt, device=dev)
         self.lam = torch.zeros((2, S, C, 4), dtype=torch.float64, device=dev)
         self.g_fnc = torch.zeros((S, C), dtype=torch.float64, device=dev)
         self.g_zbc = torch.zeros((S, C), dtype=torch.float64, device=dev)

@@ -356,7 +356,9 @@ int hydro_refresh_statics(hydro_mesh *m, hipStream_t stream);
  *   Z of the n gauges (original ids) goes to zg[k * n + i].
  * hydro_misfit_backward: the adjoint sweep of such a run.  The mesh's time
  *   must be the forward run's start time.  resid[k * n + i] = dL/dZ of gauge
- *   i after step k; states and wsums hold seg records each (hydro_adj_sizes);
+ *   i after step k; states and wsums hold 2 * seg records each
+ *   (hydro_adj_sizes): the recomputation of one segment (on a stream of the
+ *   library's, default priority) overlaps the adjoint steps of the next;
  *   lam holds two state buffers, the first of which is cleared here; g_fnc
  *   and g_zbc are accumulated (internal order).  On return *lam_index (0 or
  *   1) names the buffer holding the adjoint of the initial state. */
