@@ -133,6 +133,7 @@ struct hydro_mesh {
     DevBuf nac_own, klas_own;
     void *klas_real = nullptr;
     bool need_zb1 = false, need_slcs = false;
+    std::vector<double> xc, yc;   // cell coordinates (hydro_set_cell_coords), caller order
     hipStream_t cap_stream = nullptr;
     std::map<GraphKey, std::unique_ptr<GraphExec>> graphs;
 
@@ -263,9 +264,12 @@ template <class R> int prepare_t(hydro_mesh *m) {
     } else if (m->tune.reorder == HYDRO_REORDER_NONE) {
         perm.resize(C);
         for (int i = 0; i < C; i++) perm[i] = i;
+    } else if (m->tune.reorder == HYDRO_REORDER_HILBERT) {
+        if ((int)m->xc.size() != C)
+            return fail(HYDRO_ERR_UNSUPPORTED, "the Hilbert order needs cell coordinates: call hydro_set_cell_coords");
+        perm = hilbert_order(C, m->xc.data(), m->yc.data());
     } else {
-        return fail(HYDRO_ERR_UNSUPPORTED, "reorder %d needs cell coordinates; use RCM or NONE",
-                    m->tune.reorder);
+        return fail(HYDRO_ERR_INVALID, "reorder %d", m->tune.reorder);
     }
     std::vector<int32_t> inv(C);
     for (int p = 0; p < C; p++) inv[perm[p]] = p;
@@ -917,6 +921,8 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     if (t.block < 64 || t.block > kFusedMaxBlock || (t.block % 64))
         return fail(HYDRO_ERR_INVALID, "block %d (64..%d, multiple of 64)", t.block, kFusedMaxBlock);
     if (t.reorder < 0 || t.reorder > 3) return fail(HYDRO_ERR_INVALID, "reorder %d", t.reorder);
+    if (t.reorder == HYDRO_REORDER_HILBERT && (int)m->xc.size() != m->CELL)
+        return fail(HYDRO_ERR_UNSUPPORTED, "the Hilbert order needs cell coordinates: call hydro_set_cell_coords");
     if (t.waves_per_simd != 0 && !(t.waves_per_simd == 4 && t.scheme == HYDRO_SCHEME_TILES))
         return fail(HYDRO_ERR_INVALID, "waves_per_simd %d: 0, or 4 with the TILES scheme", t.waves_per_simd);
     if (t.scheme != HYDRO_SCHEME_TILES && t.scheme != HYDRO_SCHEME_CELL)
@@ -1089,6 +1095,32 @@ int restore_snapshot(hydro_mesh *m, const Snapshot &snap, int day, int kt, hipSt
 }
 }  // namespace
 
+int hydro_set_cell_coords(hydro_mesh *m, const double *xc, const double *yc) {
+    if (!m || (m->CELL > 0 && (!xc || !yc))) return fail(HYDRO_ERR_INVALID, "null argument");
+    for (int i = 0; i < m->CELL; i++)
+        if (!std::isfinite(xc[i]) || !std::isfinite(yc[i]))
+            return fail(HYDRO_ERR_INVALID, "cell %d has a non-finite coordinate", i);
+    m->xc.assign(xc, xc + m->CELL);
+    m->yc.assign(yc, yc + m->CELL);
+    return HYDRO_OK;
+}
+
+int hydro_plan_stats(const hydro_mesh *m, int *tiles, int *pairs, int *singles, double *edges_per_thread) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    if (!m->prepared) return fail(HYDRO_ERR_INVALID, "the mesh is not prepared");
+    const int nt = (m->CELL + m->tile - 1) / m->tile;
+    if (tiles) *tiles = nt;
+    if (pairs) *pairs = m->n_pairs;
+    if (singles) *singles = m->n_singles;
+    if (edges_per_thread) {
+        if (m->tune.scheme == HYDRO_SCHEME_TILES)
+            *edges_per_thread = nt > 0 ? (double)(m->n_pairs + m->n_singles) / ((double)nt * m->tune.block) : 0.0;
+        else
+            *edges_per_thread = 4.0;   // the CELL scheme: every thread solves its cell's four slots
+    }
+    return HYDRO_OK;
+}
+
 int hydro_autotune(hydro_mesh *m, int mode, int steps, const hydro_tune *cands, int ncand, hydro_tune *best,
                    float *us_per_step, hipStream_t s) {
     if (!m || !best || steps <= 0 || ncand < 0 || (ncand > 0 && !cands)) return fail(HYDRO_ERR_INVALID, "bad arguments");
@@ -1107,12 +1139,20 @@ int hydro_autotune(hydro_mesh *m, int mode, int steps, const hydro_tune *cands, 
             t.waves_per_simd = wps;
             c.push_back(t);
         };
+        // block x cells per tile (their ratio sets the edges per thread),
+        // the 128-VGPR build, both schemes, and the cell order
         add(256, 0, HYDRO_SCHEME_TILES, 0);
         add(256, 128, HYDRO_SCHEME_TILES, 0);
         add(128, 0, HYDRO_SCHEME_TILES, 0);
         add(256, 0, HYDRO_SCHEME_TILES, 4);
         add(128, 0, HYDRO_SCHEME_CELL, 0);
         add(256, 0, HYDRO_SCHEME_CELL, 0);
+        add(256, 0, HYDRO_SCHEME_TILES, 0);
+        c.back().reorder = HYDRO_REORDER_RCM;
+        if ((int)m->xc.size() == m->CELL) {
+            add(256, 0, HYDRO_SCHEME_TILES, 0);
+            c.back().reorder = HYDRO_REORDER_HILBERT;
+        }
     }
     const int per = m->spd - 1;
     const int64_t remaining = per > 0 ? (int64_t)m->d.NDAYS * per - global_step(m) : 0;
@@ -1992,6 +2032,16 @@ int hydro_precalculate(int dtype, int CELL, int NOD, const int32_t *NAP, void *X
                             (float *)FNC);
     else
         return fail(HYDRO_ERR_INVALID, "dtype %d", dtype);
+    return HYDRO_OK;
+}
+
+int hydro_hilbert_order(int CELL, const double *xc, const double *yc, int32_t *perm) {
+    if (CELL < 0 || (CELL > 0 && (!xc || !yc || !perm))) return fail(HYDRO_ERR_INVALID, "bad arguments");
+    for (int i = 0; i < CELL; i++)
+        if (!std::isfinite(xc[i]) || !std::isfinite(yc[i]))
+            return fail(HYDRO_ERR_INVALID, "cell %d has a non-finite coordinate", i);
+    std::vector<int32_t> p = hilbert_order(CELL, xc, yc);
+    std::memcpy(perm, p.data(), sizeof(int32_t) * CELL);
     return HYDRO_OK;
 }
 

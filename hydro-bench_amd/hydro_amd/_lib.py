@@ -76,11 +76,16 @@ class Tune(ctypes.Structure):
     ]
 
 
-def autotune_candidates():
-    """The library's built-in candidate list of hydro_autotune."""
-    return [default_tune(block=256, tile_cells=0), default_tune(block=256, tile_cells=128),
-            default_tune(block=128, tile_cells=0), default_tune(block=256, tile_cells=0, waves_per_simd=4),
-            default_tune(block=128, scheme=SCHEME_CELL), default_tune(block=256, scheme=SCHEME_CELL)]
+def autotune_candidates(hilbert=False):
+    """The library's built-in candidate list of hydro_autotune (HILBERT when
+    the mesh has cell coordinates)."""
+    c = [default_tune(block=256, tile_cells=0), default_tune(block=256, tile_cells=128),
+         default_tune(block=128, tile_cells=0), default_tune(block=256, tile_cells=0, waves_per_simd=4),
+         default_tune(block=128, scheme=SCHEME_CELL), default_tune(block=256, scheme=SCHEME_CELL),
+         default_tune(block=256, reorder=REORDER_RCM)]
+    if hilbert:
+        c.append(default_tune(block=256, reorder=REORDER_HILBERT))
+    return c
 
 
 def default_tune(**kw) -> Tune:
@@ -127,6 +132,9 @@ SIGNATURES = {
     "hydro_states_map": (c_int, [c_void_p] * 6 + [c_int, c_void_p]),
     "hydro_refresh_statics": (c_int, [c_void_p, c_void_p]),
     "hydro_autotune": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, POINTER(Tune), c_void_p, c_void_p]),
+    "hydro_set_cell_coords": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "hydro_plan_stats": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_double)]),
+    "hydro_hilbert_order": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "hydro_misfit_forward": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "hydro_misfit_backward": (c_int, [c_void_p, c_int, c_int] + [c_void_p] * 2 + [c_int] + [c_void_p] * 6
                               + [POINTER(c_int), c_void_p]),

@@ -71,6 +71,11 @@ class DeviceMesh:
         torch.cuda.synchronize(self.device)
         _lib.check(self.lib.hydro_mesh_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
+        if host.xc is not None and host.yc is not None:   # for the HILBERT cell order
+            xc = np.ascontiguousarray(host.xc, dtype=np.float64)
+            yc = np.ascontiguousarray(host.yc, dtype=np.float64)
+            _lib.check(self.lib.hydro_set_cell_coords(h, xc.ctypes.data_as(ctypes.c_void_p),
+                                                      yc.ctypes.data_as(ctypes.c_void_p)))
         if prepare:
             self.prepare(tune)
 
@@ -96,7 +101,7 @@ class DeviceMesh:
         keep the fastest; the state and time are left as they were.  Returns
         (best Tune, [(Tune, us per step), ...]); -1 marks a candidate that
         could not run."""
-        cands = list(candidates) if candidates else _lib.autotune_candidates()
+        cands = list(candidates) if candidates else _lib.autotune_candidates(hilbert=self.host.xc is not None)
         arr = (_lib.Tune * len(cands))(*cands)
         times = (ctypes.c_float * len(cands))()
         best = _lib.Tune()
@@ -104,6 +109,13 @@ class DeviceMesh:
                                            times, self.stream()))
         self.tune = best
         return best, list(zip(cands, list(times)))
+
+    def plan_stats(self):
+        """(tiles, pair tasks, one-slot tasks, edges per thread) of the prepared plan."""
+        t, p, s, e = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _lib.check(self.lib.hydro_plan_stats(self.h, ctypes.byref(t), ctypes.byref(p), ctypes.byref(s),
+                                             ctypes.byref(e)))
+        return t.value, p.value, s.value, e.value
 
     def stream(self):
         return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)

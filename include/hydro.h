@@ -125,9 +125,14 @@ typedef struct hydro_mesh_desc {
 typedef struct hydro_mesh hydro_mesh; /* opaque handle */
 
 /* Tuning of the reordered (FUSED / GRAPH / PERSISTENT) path. */
-/* Cell order of the internal layout.  TILES: compact breadth-first tiles of
- * `block` cells seeded along the RCM order, so most edges of a tile join two
- * of its own cells (the TILES scheme solves those edges once). */
+/* Cell order of the internal layout (tiles are runs of consecutive cells in
+ * it).  NONE: the caller's order.  RCM: reverse Cuthill-McKee of the cell
+ * graph.  HILBERT: cell centroids along a Hilbert space-filling curve (needs
+ * hydro_set_cell_coords).  TILES: compact breadth-first tiles of `block`
+ * cells seeded along the RCM order, so most edges of a tile join two of its
+ * own cells (the TILES scheme solves those edges once).  Every order gives
+ * the same results bit for bit: the primary side of an edge is decided on the
+ * caller's cell ids whatever the order. */
 typedef enum hydro_reorder {
     HYDRO_REORDER_NONE = 0,
     HYDRO_REORDER_RCM = 1,
@@ -183,12 +188,23 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune);
  * from the same state and time), then leaves the mesh prepared with the
  * fastest, its state and time as they were, and writes that tuning to *best.
  * cands == NULL with ncand 0 tries a built-in list over block (128, 256),
- * cells per tile, the 128-VGPR build and both schemes; us_per_step
+ * cells per tile (edges per thread), the 128-VGPR build, both schemes and the
+ * cell order (TILES, RCM, and HILBERT when coordinates are set); us_per_step
  * (optional, one entry per candidate) receives each time, -1 for a
  * candidate that could not run.  Needs `steps` steps left before the NDAYS
  * cap; synchronises the stream. */
 int hydro_autotune(hydro_mesh *m, int mode, int steps, const hydro_tune *cands, int ncand, hydro_tune *best,
                    float *us_per_step, hipStream_t stream);
+
+/* Cell centroids (HOST arrays [CELL], caller order) for the HILBERT order;
+ * no counterpart in the reference.  Copied; the next hydro_prepare uses them. */
+int hydro_set_cell_coords(hydro_mesh *m, const double *xc, const double *yc);
+
+/* The prepared plan of the TILES scheme: tiles (= workgroups of a step),
+ * edge tasks solved for both slots at once (pairs) and one-slot tasks, and
+ * edges per thread ((pairs + singles) / (tiles * block); 4 slots per thread
+ * for the CELL scheme).  The autotuner's cells-per-tile axis moves it. */
+int hydro_plan_stats(const hydro_mesh *m, int *tiles, int *pairs, int *singles, double *edges_per_thread);
 
 /* ---- the reference operator API ----------------------------------------- */
 
@@ -397,6 +413,10 @@ int hydro_scen_misfit_backward(hydro_mesh *m, int S, const double *fnc, int nste
 int hydro_precalculate(int dtype, int CELL, int NOD, const int32_t *NAP, void *XP, void *YP,
                        const void *ZBC, void *Z, const void *FNC0, double HM1, void *SIDE, void *SINF,
                        void *COSF, void *SLCOS, void *SLSIN, void *AREA, void *ZB1, void *H, void *FNC);
+
+/* Hilbert-curve order of cell centroids (host arrays): perm[new] = old.  The
+ * HILBERT layout of hydro_prepare uses it. */
+int hydro_hilbert_order(int CELL, const double *xc, const double *yc, int32_t *perm);
 
 /* Reverse Cuthill-McKee order of the cell graph of a host NAC array:
  * perm[new] = old.  The reordered layout of hydro_prepare uses it. */

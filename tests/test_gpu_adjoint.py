@@ -230,7 +230,7 @@ def _gradient_check(deep, n_dirs, fd_tol):
 
 
 def test_calibration_gradient_vs_fd_and_tlm_deep_start():
-    _gradient_check(deep=True, n_dirs=12, fd_tol=1e-6)
+    _gradient_check(deep=True, n_dirs=20, fd_tol=1e-6)   # C4: 20 random directions, h = 1e-6
 
 
 def test_calibration_gradient_vs_tlm_with_fronts():
