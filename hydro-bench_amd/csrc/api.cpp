@@ -2035,6 +2035,55 @@ int hydro_precalculate(int dtype, int CELL, int NOD, const int32_t *NAP, void *X
     return HYDRO_OK;
 }
 
+namespace {
+int check_entries(const char *what, int n, int nfiles, const int32_t *MB, const int32_t *NN, int CELL) {
+    if (n < 0 || nfiles < 0) return fail(HYDRO_ERR_INVALID, "%s: %d entries, %d files", what, n, nfiles);
+    if (n > 0 && (!MB || !NN)) return fail(HYDRO_ERR_INVALID, "%s: entry arrays missing", what);
+    for (int j = 0; j < n; j++)
+        if (MB[j] < 1 || MB[j] > CELL)
+            return fail(HYDRO_ERR_INVALID, "%s entry %d names cell %d outside 1..%d", what, j, MB[j], CELL);
+    return HYDRO_OK;
+}
+}  // namespace
+
+int hydro_take_boundary(const hydro_boundary_desc *d, const char *bounde_dir, void *QT, void *DQT, void *ZT,
+                        void *DZT, int32_t *NHQ1, void *ZW, void *QW) {
+    if (!d || !bounde_dir || !QT || !DQT || !ZT || !DZT) return fail(HYDRO_ERR_INVALID, "bad arguments");
+    if (d->dtype != HYDRO_F64 && d->dtype != HYDRO_F32) return fail(HYDRO_ERR_INVALID, "dtype %d", d->dtype);
+    if (d->CELL < 0 || d->NDAYS < 1 || d->rows < 1 || d->rows > d->NDAYS || d->MDT <= 0 || d->NHQ < 0)
+        return fail(HYDRO_ERR_INVALID, "CELL %d NDAYS %d rows %d MDT %d NHQ %d", d->CELL, d->NDAYS, d->rows, d->MDT,
+                    d->NHQ);
+    const double k0 = d->dtype == HYDRO_F64 ? (double)d->MDT / d->DT : (double)((float)d->MDT / (float)d->DT);
+    if (!(k0 >= 1.0)) return fail(HYDRO_ERR_INVALID, "MDT / DT = %g: no whole step per row", k0);
+    int st;
+    if ((st = check_entries("MBZ", d->NZ, d->NNZ0, d->MBZ, d->NNZ, d->CELL)) != HYDRO_OK) return st;
+    if ((st = check_entries("MBQ", d->NQ, d->NNQ0, d->MBQ, d->NNQ, d->CELL)) != HYDRO_OK) return st;
+    if ((st = check_entries("MBZQ", d->NZQ, d->NNZQ0, d->MBZQ, d->NNZQ, d->CELL)) != HYDRO_OK) return st;
+    if (d->NZQ > 0 && (!d->ZBC || !d->SIDE || !d->KLAS || !NHQ1 || !ZW || !QW))
+        return fail(HYDRO_ERR_INVALID, "rating curves need ZBC, SIDE, KLAS, NHQ1, ZW and QW");
+    BoundaryIn in{d->CELL, d->NDAYS, d->rows, d->MDT, d->NHQ, d->DT, d->HM2, d->NZ, d->NNZ0, d->MBZ, d->NNZ,
+                  d->NQ, d->NNQ0, d->MBQ, d->NNQ, d->NZQ, d->NNZQ0, d->MBZQ, d->NNZQ, d->ZBC, d->SIDE, d->KLAS};
+    std::string err;
+    const bool ok = d->dtype == HYDRO_F64
+                        ? take_boundary<double>(in, bounde_dir, (double *)QT, (double *)DQT, (double *)ZT,
+                                                (double *)DZT, NHQ1, (double *)ZW, (double *)QW, err)
+                        : take_boundary<float>(in, bounde_dir, (float *)QT, (float *)DQT, (float *)ZT, (float *)DZT,
+                                               NHQ1, (float *)ZW, (float *)QW, err);
+    return ok ? HYDRO_OK : fail(HYDRO_ERR_INVALID, "%s", err.c_str());
+}
+
+int hydro_boundary_increments(int dtype, int CELL, int rows_v, int rows_d, int K0, const void *V, void *D) {
+    if (CELL < 0 || rows_v < 0 || rows_d < 0 || K0 < 1 || (CELL > 0 && rows_d > 0 && (!V || !D)))
+        return fail(HYDRO_ERR_INVALID, "CELL %d rows %d/%d K0 %d", CELL, rows_v, rows_d, K0);
+    if (dtype == HYDRO_F64)
+        boundary_increments<double>(CELL, rows_v, rows_d, K0, (const double *)V, (double *)D);
+    else if (dtype == HYDRO_F32)
+        boundary_increments<float>(CELL, rows_v, rows_d, K0, (const float *)V, (float *)D);
+    else
+        return fail(HYDRO_ERR_INVALID, "dtype %d", dtype);
+    return HYDRO_OK;
+}
+
 int hydro_hilbert_order(int CELL, const double *xc, const double *yc, int32_t *perm) {
     if (CELL < 0 || (CELL > 0 && (!xc || !yc || !perm))) return fail(HYDRO_ERR_INVALID, "bad arguments");
     for (int i = 0; i < CELL; i++)

@@ -76,6 +76,19 @@ class Tune(ctypes.Structure):
     ]
 
 
+class BoundaryDesc(ctypes.Structure):
+    """hydro_boundary_desc (include/hydro.h)."""
+
+    _fields_ = [
+        ("dtype", c_int32), ("CELL", c_int32), ("NDAYS", c_int32), ("rows", c_int32), ("MDT", c_int32),
+        ("NHQ", c_int32), ("DT", c_double), ("HM2", c_double),
+        ("NZ", c_int32), ("NNZ0", c_int32), ("MBZ", c_void_p), ("NNZ", c_void_p),
+        ("NQ", c_int32), ("NNQ0", c_int32), ("MBQ", c_void_p), ("NNQ", c_void_p),
+        ("NZQ", c_int32), ("NNZQ0", c_int32), ("MBZQ", c_void_p), ("NNZQ", c_void_p),
+        ("ZBC", c_void_p), ("SIDE", c_void_p), ("KLAS", c_void_p),
+    ]
+
+
 def autotune_candidates(hilbert=False):
     """The library's built-in candidate list of hydro_autotune (HILBERT when
     the mesh has cell coordinates)."""
@@ -135,6 +148,8 @@ SIGNATURES = {
     "hydro_set_cell_coords": (c_int, [c_void_p, c_void_p, c_void_p]),
     "hydro_plan_stats": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_double)]),
     "hydro_hilbert_order": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "hydro_take_boundary": (c_int, [POINTER(BoundaryDesc), c_char_p] + [c_void_p] * 7),
+    "hydro_boundary_increments": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "hydro_misfit_forward": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "hydro_misfit_backward": (c_int, [c_void_p, c_int, c_int] + [c_void_p] * 2 + [c_int] + [c_void_p] * 6
                               + [POINTER(c_int), c_void_p]),

@@ -14,20 +14,16 @@ as cell.cpp:92-147 does):
 * per cell after a header line: PNAC, PNAP, PKLAS (``NO v0 v1 v2 v3``), PZBC,
   INITIALLEVEL, INITIALU1, INITIALV1, CV (Manning n); PXY per node
   (``NO X Y``);
-* boundary cells: MBZ / MBQ (a count of series files, then ``NO cell group``
-  per boundary cell) and BOUNDE/NZ/NZnnnn.DAT, BOUNDE/NQ/NQnnnn.DAT (a count,
-  then ``time value`` pairs).  Row i of a series is the value at
-  STIME1 = i / (24*3600/MDT) (mesh.cpp:394), linearly interpolated in the
-  last bracket [t_k, t_k+1] that contains it and 0 when none does
-  (BOUNDRYinterp; its loop also reads one entry past the end, which this
-  reader does not), rounded through a float temporary; a discharge is
-  divided by the number of cells of its group in float (mesh.cpp:442-445);
-  DZT / DQT are the row differences over K0 = MDT / DT steps, 0 on the last
-  row (mesh.cpp:451-463).
+* boundary entries: MBZ / MBQ / MBZQ (a count of series files, then
+  ``NO cell group`` per boundary cell).
 
-Geometry and the wet/dry initial state come from the library's
-``hydro_precalculate`` (MeshData::preCalculate).  Rating curves (NZQ > 0) and
-weirs (NWE > 0) are not read; a mesh that needs them raises.
+The series files under BOUNDE/ (NZ, NQ and NZQ) are read and turned into
+the per-cell QT/DQT/ZT/DZT series and the NHQ1/ZW/QW rating curves by the
+library's ``hydro_take_boundary`` (MeshData::take_boundary_for_two_d and
+BOUNDRYinterp, mesh.cpp:374-533), after the library's
+``hydro_precalculate`` (MeshData::preCalculate) has given the geometry, the
+wet/dry initial state and the side lengths the rating curves use.  Weirs
+(NWE > 0 with KLAS 6 slots) are not read; a mesh that needs them raises.
 """
 from __future__ import annotations
 
@@ -103,40 +99,6 @@ def _boundary_cells(root, name, count):
     return nfiles, out
 
 
-def _series_file(root, kind, k):
-    p = os.path.join(root, "BOUNDE", kind, f"{kind}{k:04d}.DAT")
-    with open(p, "r", encoding="latin-1") as f:
-        n = int(f.readline().split()[0])
-        pairs = []
-        for line in f:
-            parts = line.split()
-            if len(parts) >= 2:
-                pairs.append((float(parts[0]), float(parts[1])))
-            if len(pairs) == n:
-                break
-    a = np.array(pairs, dtype=np.float64).reshape(-1, 2)
-    return a[:, 0], a[:, 1]
-
-
-def boundary_interp(t, times, vals):
-    """BOUNDRYinterp: the last bracket [times[k], times[k+1]] holding t, 0
-    when none does."""
-    res = 0.0
-    for k in range(len(times) - 1):
-        if times[k] <= t <= times[k + 1]:
-            res = vals[k] + (vals[k + 1] - vals[k]) / (times[k + 1] - times[k]) * (t - times[k])
-    return res
-
-
-def _rows(times, vals, nrows, MDT):
-    """Series rows 0..nrows-1 through the float temporary."""
-    out = np.empty(nrows, dtype=np.float64)
-    for i in range(nrows):
-        t = 0.0 + i / (24.0 * 3600.0 / MDT)
-        out[i] = float(np.float32(boundary_interp(t, times, vals)))
-    return out
-
-
 def load_dat(root: str, dtype=np.float64, ndays=None, name=None) -> HostMesh:
     """Read a reference mesh directory.  `ndays` keeps only the first rows of
     the boundary series (the increments of the last kept row still come from
@@ -147,8 +109,6 @@ def load_dat(root: str, dtype=np.float64, ndays=None, name=None) -> HostMesh:
     b = [int(float(x)) for x in _scalars(root, "BOUNDARY", 6)]
     NZ, NQ, NZQ = b[0], b[1], b[2]
     NWE = b[4] if len(b) > 4 else 0
-    if NZQ > 0:
-        raise NotImplementedError("rating-curve boundaries (NZQ > 0) are not read")
     DT = float(_scalars(root, "CALTIME", 2, default=["0", "1"])[1])
     JL = float(_scalars(root, "JL", 1, default=["0"])[0])
     nac = _table(root, "PNAC", C, 5).astype(np.int64)[:, 1:]
@@ -164,44 +124,22 @@ def load_dat(root: str, dtype=np.float64, ndays=None, name=None) -> HostMesh:
     n = _table(root, "CV", C, 1)[:, 0]
 
     nd = NDAYS if ndays is None else max(1, min(int(ndays), NDAYS))
-    nrows = min(nd + 1, NDAYS)
-    K0 = int(MDT / DT)
-    R = np.dtype(dtype).type
-    QT = np.zeros(nd * C, dtype=dtype)
-    DQT = np.zeros_like(QT)
-    ZT = np.zeros_like(QT)
-    DZT = np.zeros_like(QT)
+    NHQ = b[3] if len(b) > 3 else 0
 
-    def put(cell, v, V, D):
-        v = v.astype(dtype)
-        d = np.zeros_like(v)
-        d[:-1] = (v[1:] - v[:-1]) / (R(K0) if R == np.float32 else K0)
-        if nrows == NDAYS:
-            d[-1] = 0
-        V[np.arange(nd) * C + cell] = v[:nd]
-        D[np.arange(nd) * C + cell] = d[:nd]
+    def entries(name, count):
+        nfiles, ent = _boundary_cells(root, name, count)
+        cell = np.array([c + 1 for c, _ in ent], dtype=np.int32)
+        group = np.array([g for _, g in ent], dtype=np.int32)
+        return nfiles, cell, group
 
-    nfiles, mbz = _boundary_cells(root, "MBZ", NZ)
-    for k in range(1, nfiles + 1):
-        cells = [c for c, g in mbz if g == k]
-        if not cells:
-            continue
-        v = _rows(*_series_file(root, "NZ", k), nrows, MDT)
-        for c in cells:
-            put(c, v, ZT, DZT)
-    nfiles, mbq = _boundary_cells(root, "MBQ", NQ)
-    for k in range(1, nfiles + 1):
-        cells = [c for c, g in mbq if g == k]
-        if not cells:
-            continue
-        v32 = _rows(*_series_file(root, "NQ", k), nrows, MDT).astype(np.float32) / np.float32(len(cells))
-        for c in cells:
-            put(c, v32.astype(np.float64), QT, DQT)
-
+    # the series and rating curves themselves are read by the library
+    # (hydro_take_boundary) once preCalculate has given the side lengths
+    bounde = dict(dir=os.path.join(root, "BOUNDE"), NDAYS=NDAYS, NHQ=NHQ,
+                  NZ=entries("MBZ", NZ), NQ=entries("MBQ", NQ), NZQ=entries("MBZQ", NZQ))
     raw = dict(CELL=C, NOD=NOD, XP=xy[:, 1].copy(), YP=xy[:, 2].copy(), ZBC=zbc, Zinit=z0, n=n, U0=u0, V0=v0,
                NAP=nap.reshape(-1).astype(np.int32), NAC=nac.reshape(-1).astype(np.int32),
                KLAS=klas.reshape(-1).astype(np.int32), NDAYS=nd, MDT=MDT, DT=DT, HM1=HM1, HM2=HM2, JL=JL,
-               QT=QT, DQT=DQT, ZT=ZT, DZT=DZT)
+               NHQ=NHQ, bounde=bounde)
     nv = np.where((nap[:, 3] == 0) | (nap[:, 3] == nap[:, 0]), 3, 4)
     idx = np.where(nap > 0, nap - 1, 0)
     w = (np.arange(4)[None, :] < nv[:, None]).astype(np.float64)

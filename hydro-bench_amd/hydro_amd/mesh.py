@@ -210,12 +210,44 @@ def _topology(cells, n_nodes):
 # boundary series (take_boundary_for_two_d, mesh.cpp:374-466)
 
 
-def _series(values_per_day: np.ndarray, K0: int, dtype) -> (np.ndarray, np.ndarray):
-    """Row values and per-step increments for one boundary cell."""
-    v = values_per_day.astype(dtype)
-    d = np.zeros_like(v)
-    d[:-1] = (v[1:] - v[:-1]) / dtype(K0) if dtype == np.float32 else (v[1:] - v[:-1]) / K0
-    return v, d
+def _increments(lib, V, rows, C, K0, dtype):
+    """Per-step increments of a column-major series (hydro_boundary_increments)."""
+    D = np.zeros_like(V)
+    _lib.check(lib.hydro_boundary_increments(_lib.F64 if dtype == np.float64 else _lib.F32, C, rows, rows, K0,
+                                             V.ctypes.data_as(ctypes.c_void_p), D.ctypes.data_as(ctypes.c_void_p)))
+    return D
+
+
+def _take_boundary(lib, bd, C, rows, MDT, DT, HM2, dtype, ZBC, SIDE, KLAS):
+    """Series and rating curves of a .DAT directory (hydro_take_boundary)."""
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(ctypes.c_void_p) if a.size else None
+
+    NHQ = int(bd["NHQ"])
+    d = _lib.BoundaryDesc(dtype=_lib.F64 if dtype == np.float64 else _lib.F32, CELL=C, NDAYS=int(bd["NDAYS"]),
+                          rows=rows, MDT=int(MDT), NHQ=NHQ, DT=float(DT), HM2=float(HM2))
+    for key in ("NZ", "NQ", "NZQ"):
+        nfiles, cell, group = bd[key]
+        setattr(d, key, len(cell))
+        setattr(d, "N" + key + "0", int(nfiles))
+        setattr(d, "MB" + key[1:], arr(cell))
+        setattr(d, "N" + key, arr(group))
+    d.ZBC = ZBC.ctypes.data_as(ctypes.c_void_p)
+    d.SIDE = SIDE.ctypes.data_as(ctypes.c_void_p)
+    KLAS = np.ascontiguousarray(KLAS, dtype=np.int32)
+    d.KLAS = KLAS.ctypes.data_as(ctypes.c_void_p)
+    out = {k: np.zeros(rows * C, dtype=dtype) for k in ("QT", "DQT", "ZT", "DZT")}
+    out["NHQ1"] = np.zeros(C, dtype=np.int32)
+    out["ZW"] = np.zeros(max(C * NHQ, 1), dtype=dtype)
+    out["QW"] = np.zeros(max(C * NHQ, 1), dtype=dtype)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    _lib.check(lib.hydro_take_boundary(ctypes.byref(d), os.fsencode(bd["dir"]), p(out["QT"]), p(out["DQT"]),
+                                       p(out["ZT"]), p(out["DZT"]), p(out["NHQ1"]), p(out["ZW"]), p(out["QW"])))
+    return out
 
 
 def _finish(name, raw, dtype, lib=None) -> HostMesh:
@@ -242,21 +274,22 @@ def _finish(name, raw, dtype, lib=None) -> HostMesh:
     NDAYS = raw["NDAYS"]
     K0 = int(raw["MDT"] / raw["DT"])
     QT = np.zeros(NDAYS * C, dtype=dtype)
-    DQT = np.zeros_like(QT)
     ZT = np.zeros_like(QT)
-    DZT = np.zeros_like(QT)
-    for cell, series in raw.get("q_series", {}).items():
-        v, d = _series(series, K0, R)
-        QT[np.arange(NDAYS) * C + cell] = v
-        DQT[np.arange(NDAYS) * C + cell] = d
+    for cell, series in raw.get("q_series", {}).items():   # one value per row and cell
+        QT[np.arange(NDAYS) * C + cell] = series.astype(dtype)
     for cell, series in raw.get("z_series", {}).items():
-        v, d = _series(series, K0, R)
-        ZT[np.arange(NDAYS) * C + cell] = v
-        DZT[np.arange(NDAYS) * C + cell] = d
+        ZT[np.arange(NDAYS) * C + cell] = series.astype(dtype)
+    DQT = _increments(lib, QT, NDAYS, C, K0, dtype)
+    DZT = _increments(lib, ZT, NDAYS, C, K0, dtype)
     for k in ("QT", "DQT", "ZT", "DZT"):   # series already laid out by the caller
         if k in raw:
             {"QT": QT, "DQT": DQT, "ZT": ZT, "DZT": DZT}[k][:] = raw[k].astype(dtype)
     NHQ = int(raw.get("NHQ", 0))
+    if "bounde" in raw:   # a .DAT directory: series and rating curves read by the library
+        tb = _take_boundary(lib, raw["bounde"], C, NDAYS, raw["MDT"], raw["DT"], raw["HM2"], dtype, ZBC,
+                            out["SIDE"], raw["KLAS"])
+        QT, DQT, ZT, DZT = tb["QT"], tb["DQT"], tb["ZT"], tb["DZT"]
+        raw = dict(raw, NHQ1=tb["NHQ1"], ZW=tb["ZW"], QW=tb["QW"])
     arrays = dict(
         H=outc["H"], U=raw.get("U0", np.zeros(C)).astype(dtype), V=raw.get("V0", np.zeros(C)).astype(dtype),
         Z=Z, W=np.zeros(C, dtype=dtype), ZBC=ZBC, ZB1=outc["ZB1"], AREA=outc["AREA"], FNC=outc["FNC"],

@@ -414,6 +414,59 @@ int hydro_precalculate(int dtype, int CELL, int NOD, const int32_t *NAP, void *X
                        const void *ZBC, void *Z, const void *FNC0, double HM1, void *SIDE, void *SINF,
                        void *COSF, void *SLCOS, void *SLSIN, void *AREA, void *ZB1, void *H, void *FNC);
 
+/* Inputs of hydro_take_boundary: the boundary entries of MBZ.DAT, MBQ.DAT and
+ * MBZQ.DAT as MeshData::loadFromFileMBZ/MBQ/MBZQ hold them
+ * (cuda_native_impl/hydro-cal-src/src/mesh.cpp:134-217) -- per entry j a
+ * 1-based cell id MB*[j] and the number NN*[j] of its series file -- and the
+ * scalars the series need.  Host arrays. */
+typedef struct hydro_boundary_desc {
+    int32_t dtype;      /* HYDRO_F64 or HYDRO_F32: the Real of every output */
+    int32_t CELL;
+    int32_t NDAYS;      /* rows of the series (TIME.DAT) */
+    int32_t rows;       /* rows written, 1..NDAYS; the increments of the last
+                           one still come from the row after it, if any */
+    int32_t MDT;        /* seconds per row */
+    int32_t NHQ;        /* points per rating curve: row length of ZW / QW */
+    double DT, HM2;     /* step (K0 = (int)(MDT / DT) steps per row), minimum depth */
+    int32_t NZ, NNZ0;   /* level entries, level series files NZ/NZ0001.DAT .. */
+    const int32_t *MBZ, *NNZ;
+    int32_t NQ, NNQ0;   /* discharge entries, files NQ/NQ0001.DAT .. */
+    const int32_t *MBQ, *NNQ;
+    int32_t NZQ, NNZQ0; /* rating-curve entries, files NZQ/NZQ0001.DAT .. */
+    const int32_t *MBZQ, *NNZQ;
+    const void *ZBC;    /* [CELL] bed level (rating curves only) */
+    const void *SIDE;   /* [4*CELL] side lengths, as hydro_precalculate gives them */
+    const int32_t *KLAS;/* [4*CELL] slot kinds (rating curves only) */
+} hydro_boundary_desc;
+
+/* MeshData::take_boundary_for_two_d and BOUNDRYinterp
+ * (cuda_native_impl/hydro-cal-src/src/mesh.cpp:374-533) with the expansion to
+ * cells of CellView's constructor (cell.cpp:92-126), on HOST arrays: reads
+ * <bounde_dir>/NZ/NZnnnn.DAT, NQ/NQnnnn.DAT and NZQ/NZQnnnn.DAT (a count, then
+ * `time value` or `level discharge` lines, parsed into Real) and writes
+ *   ZT, DZT, QT, DQT [rows*CELL] column-major: row jt of cell c at jt*CELL+c.
+ *     Row jt is the series at jt * MDT seconds, linearly interpolated and
+ *     rounded through float; a discharge is split evenly, in float, over the
+ *     entries of its file; D = (row jt+1 - row jt) / K0, 0 on row NDAYS-1.
+ *     Cells with no entry hold 0; of two entries of one cell the later wins.
+ *   NHQ1 [CELL], ZW, QW [CELL*NHQ] (may be NULL when NZQ == 0): per entry cell
+ *     the number of points of its curve, the levels, and its share of each
+ *     point's discharge by wetted width (depth above ZBC, at least HM2, times
+ *     the side of its last KLAS 3 slot).
+ * Every output is overwritten in full.  A missing or malformed file, an entry
+ * naming a cell outside 1..CELL or a curve longer than NHQ is
+ * HYDRO_ERR_INVALID (the reference exits instead).  Bracket search differs in
+ * one place: the reference also tests the bracket that ends one entry past a
+ * series' table, reading outside it; that bracket is not tested here. */
+int hydro_take_boundary(const hydro_boundary_desc *d, const char *bounde_dir, void *QT, void *DQT, void *ZT,
+                        void *DZT, int32_t *NHQ1, void *ZW, void *QW);
+
+/* Increments of an expanded series (host arrays, Real per dtype):
+ * D[jt*CELL+c] = (V[(jt+1)*CELL+c] - V[jt*CELL+c]) / K0 for jt < rows_d, and
+ * 0 where V has no row jt+1 (mesh.cpp:449-464).  For callers that build the
+ * series rows themselves. */
+int hydro_boundary_increments(int dtype, int CELL, int rows_v, int rows_d, int K0, const void *V, void *D);
+
 /* Hilbert-curve order of cell centroids (host arrays): perm[new] = old.  The
  * HILBERT layout of hydro_prepare uses it. */
 int hydro_hilbert_order(int CELL, const double *xc, const double *yc, int32_t *perm);

@@ -17,7 +17,8 @@ template <class R> static ExtView<R> view(const hydro_mesh_desc *d) {
     v.FNC = (const R *)d->FNC; v.BF = (const R *)d->BoundaryFeature; v.NHQ1 = d->NHQ1;
     v.ZW = (const R *)d->ZW; v.QW = (const R *)d->QW; v.QT = (const R *)d->QT; v.DQT = (const R *)d->DQT;
     v.ZT = (const R *)d->ZT; v.DZT = (const R *)d->DZT;
-    v.NAC = d->NAC; v.KLAS = d->KLAS;
+    // int32 index arrays only (index_real = 0)
+    v.NAC = (const int32_t *)d->NAC; v.KLAS = (int32_t *)d->KLAS; v.KLASr = nullptr;
     v.SIDE = (const R *)d->SIDE; v.COSF = (const R *)d->COSF; v.SINF = (const R *)d->SINF;
     v.SLCOS = (const R *)d->SLCOS; v.SLSIN = (const R *)d->SLSIN;
     v.F0 = (R *)d->FLUX0; v.F1 = (R *)d->FLUX1; v.F2 = (R *)d->FLUX2; v.F3 = (R *)d->FLUX3;
