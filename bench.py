@@ -13,7 +13,12 @@ Prints ONE JSON line:
               mesh, 1 host thread; the oracle (the C++ restatement of the
               reference kernels) timed beside it
 
-Usage: python bench.py [--gpus 1] [--steps 900] [--warmup 20] [--mode fused]
+The timed steps run, by default, as one PERSISTENT launch of the resident
+TILES kernel: 1024-thread workgroups, one tile of ~810 cells per CU, each
+workgroup keeping its tile on its CU for the whole launch.  --mode fused
+times one launch per step instead, --block / --tile other tilings.
+
+Usage: python bench.py [--gpus 1] [--steps 900] [--warmup 20] [--mode persistent]
 
 The other headline measurements run after it and are carried in the same
 line as sub-objects (--no-extras leaves them out):
@@ -50,8 +55,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=900)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--mode", default="fused", choices=["persistent", "graph", "fused", "two_kernel", "graph_two_kernel"])
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--mode", default="persistent",
+                    choices=["persistent", "graph", "fused", "two_kernel", "graph_two_kernel"])
+    ap.add_argument("--block", type=int, default=1024, help="threads per workgroup of the reordered-layout kernels")
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="cells per tile; -1: balanced, as many tiles as the GPU holds workgroups at once "
+                         "(one per CU at --block 1024)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -86,14 +95,15 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8, "copies": 5}))
 
 
-def time_gpu(host, mode, steps, warmup, block):
+def time_gpu(host, mode, steps, warmup, block, tile=0):
     import torch
 
     import hydro_amd
     from hydro_amd.device import DeviceMesh
 
-    tune = hydro_amd.default_tune(block=block)
+    tune = hydro_amd.default_tune(block=block, tile_cells=tile)
     dm = DeviceMesh(host, device=0, tune=tune)
+    tiles = dm.plan_stats()[0]
     init = host.state()
     if warmup:
         dm.step(min(warmup, host.max_steps), mode)
@@ -116,7 +126,7 @@ def time_gpu(host, mode, steps, warmup, block):
                 "graph_two_kernel": 2 * steps}[mode]
     st = dm.fetch("H")
     dm.close()
-    return ms, wall, launches, st
+    return ms, wall, launches, st, tiles
 
 
 def cpu_reference(host, steps):
@@ -447,7 +457,7 @@ def main():
 
     host64 = hydro_amd.make_config(WORKLOAD, dtype=np.float64)
     steps = min(a.steps, host64.max_steps)
-    ms, wall, launches, st = time_gpu(host64, a.mode, steps, a.warmup, a.block)
+    ms, wall, launches, st, tiles = time_gpu(host64, a.mode, steps, a.warmup, a.block, a.tile)
     if not np.all(np.isfinite(st["H"])):
         raise SystemExit("non-finite state after the timed steps")
     C = host64.CELL
@@ -459,7 +469,11 @@ def main():
     if two:
         bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C / 2.0
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-    kernel = {"persistent": "k_persistent_tiles<double,0,false>", "graph": "k_fused_step_tick<double,0,false,0>",
+    # PERSISTENT runs the resident kernel whenever every tile fits on the GPU
+    # at once (always at --block 1024 with balanced tiles), else the loop kernel
+    resident = a.mode == "persistent" and tiles <= 256 * max(1, 1024 // a.block)
+    kernel = {"persistent": "k_resident_tiles<double,0,false>" if resident else "k_persistent_tiles<double,0,false>",
+              "graph": "k_fused_step_tick<double,0,false,0>",
               "fused": "k_fused_step<double,0,false,0>", "two_kernel": "k_calculate_flux+k_update_cell",
               "graph_two_kernel": "k_calculate_flux_tick+k_update_cell"}[a.mode]
     traffic = None
@@ -481,7 +495,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded 207,234-cell floodplain mesh, DT=0.5 s)",
         "config": {"workload": WORKLOAD, "cells": C, "edges": host64.n_edges, "dt": host64.DT,
-                   "mode": a.mode, "block": a.block, "variant": "A"},
+                   "mode": a.mode, "block": a.block, "tiles": tiles,
+                   "tile_cells": "balanced" if a.tile == -1 else (a.tile or a.block), "variant": "A"},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell_step": BYTES_PER_CELL_STEP["f64"], "launches": launches,
@@ -490,7 +505,7 @@ def main():
     }
     if not a.no_fp32:
         host32 = hydro_amd.make_config(WORKLOAD, dtype=np.float32)
-        ms32, _, l32, _ = time_gpu(host32, a.mode, steps, a.warmup, a.block)
+        ms32, _, l32, _, _ = time_gpu(host32, a.mode, steps, a.warmup, a.block, a.tile)
         v32 = C * steps / (ms32 / 1e3)
         out["fp32"] = {"value": v32, "ms_per_step": ms32 / steps,
                        "roofline_frac": v32 * BYTES_PER_CELL_STEP["f32"] / 1e9 / HBM_PEAK_GBS}

@@ -112,9 +112,13 @@ struct hydro_mesh {
     DevBuf twin_kind;   // [4][C]: kind of each slot's twin slot (K_PAD none), read by the fused adjoint step
     DevBuf task_w, task_nb, task_cs, task_sd, task_n, cell_pad;   // TILES scheme task entries
     DevBuf tile_nb_off, tile_nb, tile_flag;   // persistent TILES kernel: neighbour tiles (CSR), step flags
+    DevBuf tile_pub;                          // resident kernel: wavefronts holding a tile's halo cells
     int max_tile_nb = 0;
     int tile = 256;                 // cells per tile (resolved hydro_tune.tile_cells)
     bool persist_pending = false;   // a persistent launch whose wait flag has not been read back
+    bool last_resident = false;     // the last PERSISTENT launch ran the resident TILES kernel
+    int resident_cap = -1;          // tiles the resident kernel holds at once (-1: not yet asked)
+    int flag_base = 0;              // value every tile step flag holds (between persistent TILES launches)
     int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
@@ -221,6 +225,7 @@ template <class R> FusedView<R> fused_view(const hydro_mesh *m, int in) {
     f.tile_nb_off = m->tile_nb_off.as<int32_t>();
     f.tile_nb = m->tile_nb.as<int32_t>();
     f.tile_flag = m->tile_flag.as<int32_t>();
+    f.tile_pub = m->tile_pub.as<int32_t>();
     f.gauge_map = nullptr;
     f.gauge_next = nullptr;
     f.gauge_n = 0;
@@ -271,6 +276,10 @@ template <class R> int prepare_t(hydro_mesh *m) {
     } else {
         return fail(HYDRO_ERR_INVALID, "reorder %d", m->tune.reorder);
     }
+    // TILES scheme: a tile's cells read by other tiles first, so the
+    // wavefronts holding them finish their update, and publish it, first
+    if (m->tune.scheme == HYDRO_SCHEME_TILES && m->tune.reorder != HYDRO_REORDER_NONE)
+        edge_cells_first(C, NAC.data(), m->tile, perm);
     std::vector<int32_t> inv(C);
     for (int p = 0; p < C; p++) inv[perm[p]] = p;
 
@@ -343,12 +352,11 @@ template <class R> int prepare_t(hydro_mesh *m) {
     // boundary slots).  The leftover pairs of a tile (its pair count modulo
     // 64) become two one-slot tasks each when they fit in the free entries
     // of the last one-slot chunk, which costs nothing there and saves a
-    // mostly empty pair chunk.  Chunks go to the wavefront with the least
-    // work so far, heaviest first (a one-slot chunk counts 3/2 of a pair
-    // chunk), so the wavefronts of a workgroup, which run on different
-    // SIMDs, carry near-equal work.  Every tile gets the same number of
-    // entries (its rounds times the block, the largest over tiles), so a
-    // tile's entries start at t * stride.
+    // mostly empty pair chunk.  Chunks are placed as described below, so
+    // the wavefronts of a workgroup carry near-equal work and the chunks
+    // that need other tiles' states come last.  Every tile gets the same
+    // number of entries (its rounds times the block, the largest over
+    // tiles), so a tile's entries start at t * stride.
     {
         const int B = m->tile;
         const int ntiles = (C + B - 1) / B;
@@ -364,11 +372,13 @@ template <class R> int prepare_t(hydro_mesh *m) {
             return std::memcmp(&na, &b, sizeof(R)) == 0;
         };
         // slot ids as 4*p + j during planning
-        std::vector<uint32_t> pairs, sgl, bnd;
+        std::vector<uint32_t> pairs, sgl, bnd, sgl_h, bnd_h;
         for (int t = 0; t < ntiles; t++) {
             pairs.clear();
             sgl.clear();
             bnd.clear();
+            sgl_h.clear();
+            bnd_h.clear();
             const int p0 = t * B, pend = std::min(C, (t + 1) * B);
             for (int p = p0; p < pend; p++)
                 for (int j = 0; j < 4; j++) {
@@ -378,12 +388,15 @@ template <class R> int prepare_t(hydro_mesh *m) {
                         pad_n[p] |= (uint8_t)(1u << j);
                         continue;
                     }
+                    const int n = nbr_n[sn];
+                    // a one-slot task reading a cell of another tile
+                    const bool halo = n >= 0 && (n < p0 || n >= pend);
+                    const uint32_t hb = halo ? TASK_HALO : 0u;
                     if (kind >= K_B10) {
-                        bnd.push_back(task_word(p - p0, j, 0, 0, false, kind));
+                        (halo ? bnd_h : bnd).push_back(task_word(p - p0, j, 0, 0, false, kind) | hb);
                         continue;
                     }
-                    const int n = nbr_n[sn];
-                    if (n >= p0 && n < pend && twin_n[sn] != 0xff) {
+                    if (!halo && n >= 0 && twin_n[sn] != 0xff) {
                         const int k = twin_n[sn];
                         const size_t sk = (size_t)k * C + n;
                         const int kn = meta_n[sk] & META_KIND_MASK;
@@ -400,61 +413,112 @@ template <class R> int prepare_t(hydro_mesh *m) {
                             continue;
                         }
                     }
-                    sgl.push_back(task_word(p - p0, j, 0, 0, false, kind));
+                    (halo ? sgl_h : sgl).push_back(task_word(p - p0, j, 0, 0, false, kind) | hb);
                 }
-            // leftover pairs -> one-slot tasks when the last one-slot chunk has room
+            // leftover pairs -> one-slot tasks when the last chunk of the
+            // tile's own one-slot tasks has room
             const size_t left = pairs.size() % kWave, ones = sgl.size() + bnd.size();
             const size_t room = (ones + kWave - 1) / kWave * kWave - ones;
             if (left > 0 && 2 * left <= room) {
                 std::vector<uint32_t> split;
                 for (size_t q = pairs.size() - left; q < pairs.size(); q++) {
                     const uint32_t w = pairs[q];
-                    split.push_back(task_word(w & 0xff, (w >> 8) & 3, 0, 0, false, K_PRIMARY));
-                    split.push_back(task_word((w >> 10) & 0xff, (w >> 18) & 3, 0, 0, false, K_MIRROR));
+                    split.push_back(task_word(task_la(w), task_ja(w), 0, 0, false, K_PRIMARY));
+                    split.push_back(task_word(task_lb(w), task_jb(w), 0, 0, false, K_MIRROR));
                 }
                 pairs.resize(pairs.size() - left);
                 npairs -= (int)left;
                 sgl.insert(sgl.begin(), split.begin(), split.end());
             }
+            // Boundary slots by cost of their handler (walls and open edges
+            // cheap; weirs, breaches; the fixed-point iterations of rating-
+            // curve, level and discharge edges), so the costly ones share as
+            // few chunks as possible: a wavefront runs every lane's path.
+            auto rank = [](uint32_t w) {
+                const int k = task_kind(w);
+                return (k == K_B1 || k == K_B10 || k == K_B3) ? 2 : (k == K_B6 || k == K_B7) ? 1 : 0;
+            };
+            auto by_rank = [&](uint32_t x, uint32_t y) { return rank(x) < rank(y); };
+            std::stable_sort(bnd.begin(), bnd.end(), by_rank);
+            std::stable_sort(bnd_h.begin(), bnd_h.end(), by_rank);
+            // one-slot tasks: the tile's own (interior, then boundary), then,
+            // from a chunk of their own, those that read another tile's cells
             std::vector<uint32_t> one(sgl);
             one.insert(one.end(), bnd.begin(), bnd.end());
-            nsingles += (int)one.size();
-            // chunks: {list, first entry, entries, cost}
+            const size_t nlocal = one.size(), nhalo = sgl_h.size() + bnd_h.size();
+            if (nhalo > 0) one.resize((one.size() + kWave - 1) / kWave * kWave, TASK_NONE);
+            one.insert(one.end(), sgl_h.begin(), sgl_h.end());
+            one.insert(one.end(), bnd_h.begin(), bnd_h.end());
+            nsingles += (int)(nlocal + nhalo);
+            // Chunks, in placement order: the tile's own chunks with an
+            // iterative boundary handler (the longest), pairs, its other own
+            // one-slot chunks, and last the chunks reading other tiles (the
+            // resident kernel runs every chunk that reads no other tile
+            // without waiting for the neighbouring tiles, and the others
+            // after it).  Position 0 of every wavefront takes the first
+            // chunks, so each long chunk gets a wavefront of its own; the
+            // others fill position 1, 2, ... of the remaining wavefronts in
+            // an order that visits the four SIMD groups in turn (wavefronts
+            // w, w+4, w+8, w+12 share a SIMD), group 0 last, as it holds the
+            // wavefront with the tile's last, partial row of cells (or one
+            // more cell row than the others).
             struct Chunk {
                 const std::vector<uint32_t> *src;
                 size_t at, n;
-                int cost;
+                int cls;   // 0 long own, 1 pairs, 2 other own, 3 reads other tiles
             };
             std::vector<Chunk> ch;
             for (size_t q = 0; q < pairs.size(); q += kWave)
-                ch.push_back({&pairs, q, std::min<size_t>(kWave, pairs.size() - q), 2});
-            for (size_t q = 0; q < one.size(); q += kWave)
-                ch.push_back({&one, q, std::min<size_t>(kWave, one.size() - q), 3});
-            std::stable_sort(ch.begin(), ch.end(), [](const Chunk &a, const Chunk &b) { return a.cost > b.cost; });
-            std::vector<int> load(nwave, 0), cnt(nwave, 0);
-            std::vector<std::pair<int, const Chunk *>> placed;   // (wavefront, chunk)
-            for (const Chunk &c : ch) {
-                int w = 0;
-                for (int v = 1; v < nwave; v++)
-                    if (load[v] < load[w] || (load[v] == load[w] && cnt[v] < cnt[w])) w = v;
-                placed.push_back({w, &c});
-                load[w] += c.cost;
-                cnt[w]++;
+                ch.push_back({&pairs, q, std::min<size_t>(kWave, pairs.size() - q), 1});
+            for (size_t q = 0; q < one.size(); q += kWave) {
+                const size_t n = std::min<size_t>(kWave, one.size() - q);
+                bool halo = false, heavy = false;
+                for (size_t e = q; e < q + n; e++)
+                    if (one[e] != TASK_NONE) {
+                        halo = halo || (one[e] & TASK_HALO);
+                        heavy = heavy || rank(one[e]) == 2;
+                    }
+                ch.push_back({&one, q, n, halo ? 3 : heavy ? 0 : 2});
             }
-            // wavefronts with more chunks first, so the unused positions are
-            // the last ones and a tile's entries end at its used positions
-            std::vector<int> by(nwave), slot(nwave), next(nwave, 0);
-            for (int v = 0; v < nwave; v++) by[v] = v;
-            std::stable_sort(by.begin(), by.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
-            for (int v = 0; v < nwave; v++) slot[by[v]] = v;
+            std::stable_sort(ch.begin(), ch.end(), [](const Chunk &x, const Chunk &y) { return x.cls < y.cls; });
             int used = 0;
             std::vector<uint32_t> &L = lists[t];
             L.clear();
-            for (const auto &pc : placed) {
-                const int w = pc.first, pos = next[w]++ * nwave + slot[w];
+            auto put = [&](int pos, const Chunk &c) {
                 used = std::max(used, pos + 1);
                 if (L.size() < (size_t)(pos + 1) * kWave) L.resize((size_t)(pos + 1) * kWave, TASK_NONE);
-                std::copy_n(pc.second->src->begin() + pc.second->at, pc.second->n, L.begin() + (size_t)pos * kWave);
+                std::copy_n(c.src->begin() + c.at, c.n, L.begin() + (size_t)pos * kWave);
+            };
+            size_t next = 0;
+            std::vector<char> alone(nwave, 0);   // wavefronts running one long chunk and nothing else
+            for (int w = 0; w < nwave && next < ch.size(); w++) {
+                alone[w] = ch[next].cls == 0 && w + 1 < nwave;
+                put(w, ch[next++]);
+            }
+            std::vector<int> order;   // wavefront order of positions 1, 2, ...
+            for (int base = 0; base < nwave; base += 4)
+                for (int g : {1, 2, 3, 0})
+                    if (base + g < nwave && !alone[base + g]) order.push_back(base + g);
+            if (order.empty()) order.push_back(nwave - 1);
+            // chunks reading other tiles go to wavefronts without cells of
+            // their own first (their update pass is the shortest), then to
+            // the first wavefronts
+            const int cellw = (pend - p0 + (int)kWave - 1) / (int)kWave;
+            std::vector<int> horder;
+            for (int w = nwave - 1; w >= cellw; w--)
+                if (!alone[w]) horder.push_back(w);
+            for (int w = 0; w < cellw; w++)
+                if (!alone[w]) horder.push_back(w);
+            if (horder.empty()) horder = order;
+            std::vector<int> hnext(nwave, 1);   // next free position per wavefront
+            for (int round = 1; next < ch.size() && ch[next].cls < 3; round++)
+                for (size_t k = 0; k < order.size() && next < ch.size() && ch[next].cls < 3; k++) {
+                    put(round * nwave + order[k], ch[next++]);
+                    hnext[order[k]] = round + 1;
+                }
+            for (size_t k = 0; next < ch.size(); k++) {
+                const int w = horder[k % horder.size()];
+                put(hnext[w]++ * nwave + w, ch[next++]);
             }
             used_n[t] = used;
             most = std::max(most, L.size());
@@ -473,15 +537,26 @@ template <class R> int prepare_t(hydro_mesh *m) {
                 const uint32_t w = lists[t][q];
                 if (w == TASK_NONE) continue;
                 const size_t k = (size_t)t * stride + q;
-                const int pa = p0 + (int)(w & 0xff), ja = (int)((w >> 8) & 3);
+                const int pa = p0 + task_la(w), ja = task_ja(w);
                 const size_t ea = (size_t)ja * C + pa;
                 tw[k] = w;
                 tnb[k] = nbr_n[ea];
                 tcs[2 * k] = cos_n[ea];
                 tcs[2 * k + 1] = sin_n[ea];
                 tsd[k] = side_n[ea];   // a pair's two slots have the same SIDE (a pairing condition)
+                // a cell another tile's one-slot task reads is a halo cell
+                // (bit 4): the resident kernel publishes only those every step
+                const int nb = nbr_n[ea];
+                if (!(w & TASK_PAIR) && nb >= 0 && nb / B != t) pad_n[nb] |= (uint8_t)kCellHalo;
             }
         }
+        // wavefronts of each tile up to its last cell that other tiles read
+        // (kCellHalo): the resident kernel publishes the tile's step once
+        // they have stored their cells
+        std::vector<int32_t> pub(ntiles, 0);
+        for (int p = 0; p < C; p++)
+            if (pad_n[p] & kCellHalo) pub[p / B] = std::max(pub[p / B], (p - (p / B) * B) / (int)kWave + 1);
+        if ((rc = h2d(m->tile_pub, pub))) return rc;
         m->n_pairs = npairs;
         m->n_singles = nsingles;
         m->task_stride = (int)stride;
@@ -514,6 +589,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
         if ((rc = h2d(m->tile_nb_off, off)) || (rc = h2d(m->tile_nb, lst)) ||
             (rc = m->tile_flag.alloc((size_t)nt * sizeof(int32_t))))
             return rc;
+        HIP_TRY(hipMemset(m->tile_flag.p, 0, (size_t)nt * sizeof(int32_t)));
+        m->flag_base = 0;
     }
     if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
         (rc = h2d(m->DTA, dta_n)) ||
@@ -578,6 +655,9 @@ int check_persistent(hydro_mesh *m, hipStream_t s) {
     HIP_TRY(hipMemcpy(&err, m->perr.p, sizeof(int), hipMemcpyDeviceToHost));
     if (err) {
         HIP_TRY(hipMemset(m->perr.p, 0, sizeof(int)));
+        // the tiles stopped at different steps: start the flags afresh
+        if (m->tile_flag.p) HIP_TRY(hipMemset(m->tile_flag.p, 0, m->tile_flag.bytes));
+        m->flag_base = 0;
         return fail(HYDRO_ERR_HIP, "a persistent launch stopped: a wait between steps exceeded its spin limit");
     }
     return HYDRO_OK;
@@ -742,18 +822,8 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
     if (rc || (rc = ensure_internal<R>(m, s))) return rc;
     if (n == 0) return HYDRO_OK;
     const int block = m->tune.block;
-    int maxg = 0;
-    HIP_TRY((persistent_max_grid<R, VAR>(block, m->rare, m->tune.scheme, m->tune.waves_per_simd == 4, &maxg)));
     const int ntiles = (m->CELL + m->tile - 1) / m->tile;
-    int want = m->tune.persistent_wgs_per_cu;
-    int grid = maxg;
-    if (want > 0) {
-        int dev = 0, cus = 0;
-        HIP_TRY(hipGetDevice(&dev));
-        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid = std::min(maxg, want * cus);
-    }
-    grid = std::max(1, std::min(grid, ntiles));
+    const int want = m->tune.persistent_wgs_per_cu;
     const FusedView<R> f = fused_view<R>(m, m->cur);
     PersistentArgs pa;
     pa.nsteps = n;
@@ -763,10 +833,43 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
     pa.barrier = m->barrier.as<unsigned>();
     pa.error = m->perr.as<int>();
     pa.spin_limit = m->tune.spin_limit > 0 ? (unsigned)m->tune.spin_limit : 1u << 24;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
-    HIP_TRY(hipGetLastError());
-    if (f.scheme == HYDRO_SCHEME_TILES) HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles, s));
-    HIP_TRY((launch_persistent<R, VAR>(f, pa, block, grid, s)));
+    pa.flag0 = 0;
+    // the resident kernel when every tile fits on the GPU at once (the
+    // capacity is asked once per prepared layout: nothing but the launch
+    // itself is left on the per-call path)
+    bool resident = false;
+    if (f.scheme == HYDRO_SCHEME_TILES && !m->tune.persistent_loop && want == 0) {
+        if (m->resident_cap < 0) HIP_TRY((resident_capacity<R, VAR>(block, m->rare, &m->resident_cap)));
+        resident = m->resident_cap >= f.ntiles;
+    }
+    if (resident) {
+        // the flags need no reset: this launch counts on from where the
+        // last one left them (restarting from 0 long before int32 overflow)
+        if ((int64_t)m->flag_base + n > (1 << 30)) {
+            HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles, s));
+            m->flag_base = 0;
+        }
+        pa.flag0 = m->flag_base;
+        HIP_TRY((launch_resident<R, VAR>(f, pa, m->tune.xcd_remap, s)));
+        m->flag_base += n;
+    } else {
+        int maxg = 0;
+        HIP_TRY((persistent_max_grid<R, VAR>(block, m->rare, m->tune.scheme, m->tune.waves_per_simd == 4, &maxg)));
+        int grid = maxg;
+        if (want > 0) {
+            int dev = 0, cus = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            grid = std::min(maxg, want * cus);
+        }
+        grid = std::max(1, std::min(grid, ntiles));
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
+        HIP_TRY(hipGetLastError());
+        if (f.scheme == HYDRO_SCHEME_TILES) HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles, s));
+        HIP_TRY((launch_persistent<R, VAR>(f, pa, block, grid, s)));
+        m->flag_base = (f.scheme == HYDRO_SCHEME_TILES) ? n : m->flag_base;
+    }
+    m->last_resident = resident;
     m->persist_pending = true;
     if (n & 1) m->cur ^= 1;
     for (int i = 0; i < n; i++) advance(m);
@@ -798,6 +901,7 @@ hydro_tune default_tune() {
     t.scheme = HYDRO_SCHEME_TILES;
     t.tile_cells = 0;
     t.spin_limit = 0;
+    t.persistent_loop = 0;
     return t;
 }
 
@@ -918,8 +1022,12 @@ int hydro_mesh_destroy(hydro_mesh *m) {
 int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
     hydro_tune t = tune ? *tune : default_tune();
-    if (t.block < 64 || t.block > kFusedMaxBlock || (t.block % 64))
-        return fail(HYDRO_ERR_INVALID, "block %d (64..%d, multiple of 64)", t.block, kFusedMaxBlock);
+    {
+        const int most = (t.scheme == HYDRO_SCHEME_TILES) ? kTileMaxBlock : kFusedMaxBlock;
+        if (t.block < 64 || t.block > most || (t.block % 64))
+            return fail(HYDRO_ERR_INVALID, "block %d (64..%d, multiple of 64, for scheme %d)", t.block, most,
+                        t.scheme);
+    }
     if (t.reorder < 0 || t.reorder > 3) return fail(HYDRO_ERR_INVALID, "reorder %d", t.reorder);
     if (t.reorder == HYDRO_REORDER_HILBERT && (int)m->xc.size() != m->CELL)
         return fail(HYDRO_ERR_UNSUPPORTED, "the Hilbert order needs cell coordinates: call hydro_set_cell_coords");
@@ -927,12 +1035,25 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
         return fail(HYDRO_ERR_INVALID, "waves_per_simd %d: 0, or 4 with the TILES scheme", t.waves_per_simd);
     if (t.scheme != HYDRO_SCHEME_TILES && t.scheme != HYDRO_SCHEME_CELL)
         return fail(HYDRO_ERR_INVALID, "scheme %d", t.scheme);
-    if (t.tile_cells != 0 && (t.tile_cells < 64 || t.tile_cells > t.block))
-        return fail(HYDRO_ERR_INVALID, "tile_cells %d (0, or 64..block)", t.tile_cells);
+    if (t.tile_cells != 0 && t.tile_cells != HYDRO_TILE_BALANCED && (t.tile_cells < 64 || t.tile_cells > t.block))
+        return fail(HYDRO_ERR_INVALID, "tile_cells %d (0, -1, or 64..block)", t.tile_cells);
+    if (t.tile_cells == HYDRO_TILE_BALANCED && t.scheme != HYDRO_SCHEME_TILES)
+        return fail(HYDRO_ERR_INVALID, "tile_cells -1 (balanced) needs the TILES scheme");
     if (t.spin_limit < 0) return fail(HYDRO_ERR_INVALID, "spin_limit %d", t.spin_limit);
+    if (t.persistent_loop != 0 && t.persistent_loop != 1)
+        return fail(HYDRO_ERR_INVALID, "persistent_loop %d (0 or 1)", t.persistent_loop);
     int tile = t.block;
     if (t.scheme == HYDRO_SCHEME_TILES && t.tile_cells > 0) {
         tile = t.tile_cells;
+    } else if (t.tile_cells == HYDRO_TILE_BALANCED) {
+        // as many tiles as the GPU holds workgroups of `block` threads at
+        // once (16 waves per CU), every CU the same number, so no CU carries
+        // one tile more than the others; at most `block` cells per tile
+        int dev = 0, cus = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int64_t slots = (int64_t)std::max(1, cus) * std::max(1, kTileMaxBlock / t.block);
+        tile = (int)std::min<int64_t>(t.block, std::max<int64_t>(64, ((int64_t)m->CELL + slots - 1) / slots));
     } else if (t.scheme == HYDRO_SCHEME_TILES) {
         // a mesh with fewer full tiles than CUs leaves CUs idle: cut it into
         // one tile per CU instead (a tile's time is mostly the latency of its
@@ -951,6 +1072,7 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     if (rc) return rc;
     HIP_TRY(hipDeviceSynchronize());
     m->tune = t;
+    m->resident_cap = -1;   // block and rare may have changed
     m->tile = tile;
     m->prepared = false;
     return (m->d.dtype == HYDRO_F64) ? prepare_t<double>(m) : prepare_t<float>(m);
@@ -1918,6 +2040,7 @@ template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hip
             PersistentArgs pa;
             pa.nsteps = n;
             pa.day0 = pa.kt0 = pa.spd = 0;   // per-mesh time comes from the batch tables
+            pa.flag0 = 0;
             pa.barrier = b->barrier.as<unsigned>();
             pa.error = b->perr.as<int>();
             pa.spin_limit = b->spin_limit;
@@ -1944,12 +2067,14 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
     auto b = std::make_unique<hydro_batch>();
     b->dtype = meshes[0]->d.dtype;
     b->variant = meshes[0]->d.variant;
+    if (t.block < 64 || t.block > kFusedMaxBlock || (t.block % 64))
+        return fail(HYDRO_ERR_INVALID, "batch block %d (64..%d, multiple of 64)", t.block, kFusedMaxBlock);
     b->block = t.block;
     b->remap = t.xcd_remap;
     b->chunk = t.graph_chunk;
     b->scheme = t.scheme;
     if (t.spin_limit > 0) b->spin_limit = (unsigned)t.spin_limit;
-    if (t.tile_cells == 0) t.tile_cells = t.block;   // members keep whole tiles: the batch balances them
+    if (t.tile_cells <= 0) t.tile_cells = t.block;   // members keep whole tiles: the batch balances them
     std::vector<int32_t> tile_mesh, tile0(n);
     for (int i = 0; i < n; i++) {
         hydro_mesh *m = meshes[i];

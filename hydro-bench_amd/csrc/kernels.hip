@@ -10,7 +10,10 @@
 //  k_persistent : the fused step for many steps in one launch, with a
 //      grid-wide barrier between steps (the design of
 //      cuda_impl/F2_driver.cu:34-44, written natively).
+#include <mutex>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "ext_step.hpp"
 #include "kernels.hpp"
@@ -285,13 +288,13 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
                                          const State<R> *__restrict__ Sin, int B, int p0, int kt, int jt) {
     const Scalars<R> S = f.S;
     const int C = f.C;
-    const int la = t.w & 0xff, ja = (t.w >> 8) & 3;
+    const int la = task_la(t.w), ja = task_ja(t.w);
     const int pa = p0 + la;
     const State<R> A = L.st[la];
     const R zbA = L.zb[la];
     const R zb1A = f.ZB1 ? L.zb1[la] : (R)(zbA + S.HM1);
     if (t.w & TASK_PAIR) {
-        const int lb = (t.w >> 10) & 0xff, jb = (t.w >> 18) & 3;
+        const int lb = task_lb(t.w), jb = task_jb(t.w);
         const int pb = p0 + lb;
         const State<R> M = L.st[lb];
         const R zbM = L.zb[lb];
@@ -303,7 +306,7 @@ __device__ __forceinline__ void task_run(const FusedView<R> &f, const TileLds<R>
         tile_store<R>(f, L, B, lb, jb, pb, fM, t.sb, -t.c, -t.s);
         return;
     }
-    int kind = (t.w >> 21) & 0xf;
+    int kind = task_kind(t.w);
     int mt = 0;
     const int ea = ja * C + pa;
     if (RARE && kind == K_B7) {   // a breach may have opened: its kind is in meta
@@ -349,6 +352,11 @@ struct NoWait {
 // before any neighbour state is read (persistent launches wait there for the
 // neighbouring tiles' previous step); false ends the tile at once, and
 // tile_cells returns false.
+template <class R, int VAR, bool RARE, bool SCEN>
+__device__ __forceinline__ State<R> tile_finish(const FusedView<R> &f, const TileLds<R> &L, int B, int i, int p,
+                                                const State<R> &me, R zbp, R area, R fnc, uint32_t pad,
+                                                const R (&sd)[4], int scen);
+
 template <class R, int VAR, bool RARE, bool COH = false, bool SCEN = false, class Wait = NoWait>
 __device__ __forceinline__ bool tile_cells(const FusedView<R> &f, const State<R> *__restrict__ Sin,
                                            State<R> *__restrict__ Sout, int t, int kt, int jt, int scen = 0,
@@ -413,7 +421,19 @@ __device__ __forceinline__ bool tile_cells(const FusedView<R> &f, const State<R>
     }
     __syncthreads();
     if (!live) return true;
-    // Phase 3: the cell's four slots in slot order, then its update.
+    st_state<COH>(Sout, p, C, tile_finish<R, VAR, RARE, SCEN>(f, L, B, i, p, me, zbp, area, fnc, pad, sd, scen));
+    return true;
+}
+
+// Phase 3 of a tile step for the thread of live cell p (local index i): the
+// cell's four slots in slot order from the LDS terms, then its update; the
+// new state is returned (W = speed(U, V) is formed on the way out).
+template <class R, int VAR, bool RARE, bool SCEN>
+__device__ __forceinline__ State<R> tile_finish(const FusedView<R> &f, const TileLds<R> &L, int B, int i, int p,
+                                                const State<R> &me, R zbp, R area, R fnc, uint32_t pad,
+                                                const R (&sd)[4], int scen) {
+    const Scalars<R> S = f.S;
+    const int C = f.C;
     const R zb1 = f.ZB1 ? L.zb1[i] : (R)(zbp + S.HM1);
     R WH = 0.0, WU = 0.0, WV = 0.0;
 #pragma unroll
@@ -448,8 +468,7 @@ __device__ __forceinline__ bool tile_cells(const FusedView<R> &f, const State<R>
     const CellOut<R> o =
         (VAR == 0) ? update_with_dta<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV)
                    : update_math<R, VAR>(S, me.H, me.U, me.V, zbp, area, fnc, WH, WU, WV, sd[0], sd[1], sd[2], sd[3]);
-    st_state<COH>(Sout, p, C, State<R>{o.H, o.U, o.V, o.Z});   // W = speed(U, V) is formed on the way out
-    return true;
+    return State<R>{o.H, o.U, o.V, o.Z};
 }
 
 // One step of tile t under either scheme.
@@ -465,7 +484,8 @@ __device__ __forceinline__ void step_tile(const FusedView<R> &f, const State<R> 
 }
 
 template <class R, int VAR, bool RARE, int SCHEME>
-__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt, int jt, int remap) {
+__global__ void __launch_bounds__(SCHEME == 0 ? kTileMaxBlock : kFusedMaxBlock) k_fused_step(FusedView<R> f, int kt,
+                                                                                           int jt, int remap) {
     step_tile<R, VAR, RARE, SCHEME>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
 
@@ -473,7 +493,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step(FusedView<R> f, i
 // scenario w / ntiles.  The scenarios share the mesh's geometry and tasks and
 // differ in state and Manning field (FNC [S][C]); each reads its own gauges.
 template <class R, int VAR, bool RARE>
-__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_scen(FusedView<R> f, int kt, int jt, int remap) {
+__global__ void __launch_bounds__(kTileMaxBlock) k_fused_step_scen(FusedView<R> f, int kt, int jt, int remap) {
     const int scen = blockIdx.x / f.ntiles, wg = blockIdx.x - scen * f.ntiles;
     const size_t so = (size_t)scen * f.C;
     tile_cells<R, VAR, RARE, false, true>(f, f.Sin + so, f.Sout + so, tile_of(wg, f.ntiles, remap), kt, jt, scen);
@@ -482,7 +502,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_scen(FusedView<R>
 // TILES scheme limited to 128 VGPRs: four waves per SIMD (LDS allows four
 // 256-thread workgroups per CU), for a 207K-cell mesh one resident round.
 template <class R, int VAR, bool RARE>
-__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kTileMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_fused_step_o4(FusedView<R> f, int kt, int jt, int remap) {
     tile_cells<R, VAR, RARE>(f, f.Sin, f.Sout, tile_of(blockIdx.x, f.ntiles, remap), kt, jt);
 }
@@ -507,7 +527,8 @@ __device__ __forceinline__ void clock_at(StepClock c, int k, int spd, int *day, 
 // the same captured graph advances the boundary series on every replay (the
 // reference's graph replays one frozen (kt=1, day=0) step, benchmark.cu:235-240).
 template <class R, int VAR, bool RARE, int SCHEME>
-__global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_tick(FusedView<R> f, const StepClock *clock,
+__global__ void __launch_bounds__(SCHEME == 0 ? kTileMaxBlock : kFusedMaxBlock) k_fused_step_tick(FusedView<R> f,
+                                                                                                const StepClock *clock,
                                                                     int offset, int spd, int remap) {
     int day, kt;
     clock_at(*clock, offset, spd, &day, &kt);
@@ -515,7 +536,7 @@ __global__ void __launch_bounds__(kFusedMaxBlock) k_fused_step_tick(FusedView<R>
 }
 
 template <class R, int VAR, bool RARE>
-__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kTileMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_fused_step_tick_o4(FusedView<R> f, const StepClock *clock, int offset, int spd, int remap) {
     int day, kt;
     clock_at(*clock, offset, spd, &day, &kt);
@@ -593,12 +614,15 @@ template <class R>
 __device__ __forceinline__ bool wait_tiles(const FusedView<R> &f, int t, int target, const PersistentArgs &pa) {
     const int a = f.tile_nb_off[t], b = f.tile_nb_off[t + 1];
     int ok = 1;
+    // one lane per neighbouring tile polls its flag back to back (a poll
+    // is one device-coherent load, whose latency paces the loop); every 64th
+    // poll also checks the spin limit and whether another tile gave up
     for (int k = a + (int)threadIdx.x; k < b; k += blockDim.x) {
         const int u = f.tile_nb[k];
         unsigned spins = 0;
         while (__hip_atomic_load(f.tile_flag + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > pa.spin_limit || __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if ((++spins & 63) == 0 &&
+                (spins > pa.spin_limit || __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -606,6 +630,30 @@ __device__ __forceinline__ bool wait_tiles(const FusedView<R> &f, int t, int tar
         }
     }
     return __syncthreads_and(ok) != 0;
+}
+
+// The same wait by the lanes of one wavefront only (lane = thread index
+// mod 64): no workgroup barrier, so the other wavefronts of the tile go on.
+// Wavefront-uniform result.
+template <class R>
+__device__ __forceinline__ bool wave_wait_tiles(const FusedView<R> &f, int t, int target, const PersistentArgs &pa,
+                                                int lane) {
+    const int a = f.tile_nb_off[t], b = f.tile_nb_off[t + 1];
+    bool ok = true;
+    for (int k = a + lane; k < b; k += 64) {
+        const int u = f.tile_nb[k];
+        unsigned spins = 0;
+        while (__hip_atomic_load(f.tile_flag + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if ((++spins & 63) == 0 &&
+                (spins > pa.spin_limit || __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = false;
+                break;
+            }
+        }
+        if (!ok) break;
+    }
+    return __builtin_amdgcn_ballot_w64(!ok) == 0;
 }
 
 template <class R, int VAR, bool RARE>
@@ -630,19 +678,190 @@ __device__ __forceinline__ void persistent_tiles(const FusedView<R> &f, const Pe
 }
 
 template <class R, int VAR, bool RARE>
-__global__ void __launch_bounds__(kFusedMaxBlock) k_persistent_tiles(FusedView<R> f, PersistentArgs pa) {
+__global__ void __launch_bounds__(kTileMaxBlock) k_persistent_tiles(FusedView<R> f, PersistentArgs pa) {
     persistent_tiles<R, VAR, RARE>(f, pa);
 }
 
 // the same limited to 128 VGPRs (four workgroups per CU, some spills)
 template <class R, int VAR, bool RARE>
-__global__ void __launch_bounds__(kFusedMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kTileMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_persistent_tiles_o4(FusedView<R> f, PersistentArgs pa) {
     persistent_tiles<R, VAR, RARE>(f, pa);
 }
 
+// Neighbour of a task in the resident kernel: a cell of the tile comes from
+// LDS (its state of this step until the tile's update), another tile's from
+// the coherent buffer `a`, which holds it for every halo cell.
+template <class R>
+__device__ __forceinline__ void resident_neighbour(const FusedView<R> &f, const TileLds<R> &L,
+                                                   const State<R> *__restrict__ a, TaskIn<R> &t, int p0, int own) {
+    const bool single = t.w != TASK_NONE && !(t.w & TASK_PAIR);
+    const int ni = (single && t.nb >= 0) ? t.nb : own;
+    const int l = ni - p0;
+    if (l >= 0 && l < f.tile) {
+        t.on = L.st[l];
+        t.zbn = L.zb[l];
+    } else {
+        t.on = ld_state<true>(a, ni, f.C);
+        t.zbn = f.ZBC[ni];
+    }
+}
+
+template <class R> __device__ __forceinline__ void hold(TaskIn<R> &t) {
+    asm volatile("" : "+v"(t.w), "+v"(t.nb), "+v"(t.c), "+v"(t.s), "+v"(t.sa));
+    t.sb = t.sa;
+}
+
+// Resident TILES kernel: one workgroup per tile for the whole launch (the
+// grid is exactly the tile count, all resident), so nothing a tile owns
+// leaves the CU between steps.  The tile's statics and the task entries of
+// its first two positions stay in registers, its cell states in LDS.  A step
+// runs the chunks that read only the tile's own cells first (pairs, and
+// one-slot tasks whose neighbour is in the tile, from LDS), then waits for
+// the neighbouring tiles' previous step (wait_tiles, as persistent_tiles),
+// then runs the chunks that read other tiles' halo cells (from the coherent
+// ping-pong buffers), then the sums and update.  The new state goes into LDS
+// for this tile's next step and, for the halo cells (and every cell at the
+// last step), to the step's output buffer.  Same arithmetic as tile_cells,
+// so the same bits.
+template <class R, int VAR, bool RARE>
+__device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const PersistentArgs &pa, int remap) {
+    const int t = tile_of(blockIdx.x, f.ntiles, remap);
+    const int B = f.block, C = f.C;
+    const int p0 = t * f.tile;
+    const TileLds<R> L = tile_lds<R>(B);
+    R zbp, area, fnc;
+    uint32_t pad;
+    R sd[4] = {0, 0, 0, 0};
+    TaskIn<R> t0 = none_task<R>(), t1 = none_task<R>();
+    const int k0 = t * f.task_stride, kn = k0 + 64 * f.task_n[t];
+    // the first npub wavefronts hold every cell other tiles read: once they
+    // have stored their cells' new states the step is published
+    const int npub = f.tile_pub[t];
+    __shared__ unsigned lds_pub;   // publishing wavefronts counted in so far
+    if (threadIdx.x == 0) lds_pub = 0;
+    {
+        const int i = threadIdx.x, p = p0 + i;
+        const bool live = i < f.tile && p < C;
+        const int pc = live ? p : C - 1;
+        const State<R> me = ld_state<true>(f.Sin, pc, C);
+        zbp = f.ZBC[pc];
+        const R z1 = (f.ZB1 ? f.ZB1 : f.ZBC)[pc];
+        area = (VAR == 0) ? f.DTA[pc] : f.AREA[pc];
+        fnc = f.FNC[pc];
+        pad = f.cell_pad[pc];
+        if (VAR == 1) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) sd[j] = f.SIDE[j * C + pc];
+        }
+        const int kw = k0 + (i & ~63);
+        if (kw < kn) t0 = task_load<R>(f, k0 + i);
+        if (kw + B < kn) t1 = task_load<R>(f, k0 + i + B);
+        if (live) {
+            L.st[i] = me;
+            L.zb[i] = zbp;
+            if (f.ZB1) L.zb1[i] = z1;
+        }
+    }
+    // entries past the first two positions are loaded every step; their
+    // wavefronts wait for the neighbouring tiles as those with halo chunks
+    const bool extra = kn > k0 + 2 * B;
+    const bool any_wait = __syncthreads_or((t0.w != TASK_NONE && (t0.w & TASK_HALO)) ||
+                                           (t1.w != TASK_NONE && (t1.w & TASK_HALO)) || extra) != 0;
+    // wavefront 0 waits when it has such chunks itself, or when no
+    // wavefront of the tile has any
+    const bool wait_any = !any_wait || __builtin_amdgcn_ballot_w64((t0.w != TASK_NONE && (t0.w & TASK_HALO)) ||
+                                                                   (t1.w != TASK_NONE && (t1.w & TASK_HALO)) ||
+                                                                   extra) != 0;
+    const int64_t g0 = (int64_t)pa.day0 * (pa.spd - 1) + (pa.kt0 - 1);
+    for (int s = 0; s < pa.nsteps; s++) {
+        // The thread index and the values held across steps pass through an
+        // empty asm every step, so nothing derived from them (LDS offsets,
+        // the slots' SIDE*COSF products, ...) is hoisted out of the loop into
+        // registers held across the whole step.
+        int i = threadIdx.x;
+        asm volatile("" : "+v"(i));
+        hold(t0);
+        hold(t1);
+        asm volatile("" : "+v"(zbp), "+v"(area), "+v"(fnc), "+v"(pad));
+        // chunks reading only the tile's own cells run before the wait, those
+        // reading other tiles' cells (TASK_HALO) after it
+        const bool pre0 = t0.w != TASK_NONE && !(t0.w & TASK_HALO);
+        const bool pre1 = t1.w != TASK_NONE && !(t1.w & TASK_HALO);
+        const bool post0 = t0.w != TASK_NONE && (t0.w & TASK_HALO);
+        const bool post1 = t1.w != TASK_NONE && (t1.w & TASK_HALO);
+        const int p = p0 + i;
+        const bool live = i < f.tile && p < C;
+        const int pc = live ? p : C - 1;
+        int day, kt;
+        time_of(g0 + s, pa.spd, &day, &kt);
+        const State<R> *a = (s & 1) ? f.Sout : f.Sin;
+        State<R> *b = (s & 1) ? const_cast<State<R> *>(f.Sin) : f.Sout;
+        if (pre0) {
+            if (!(t0.w & TASK_PAIR)) resident_neighbour(f, L, a, t0, p0, pc);
+            task_run<R, VAR, RARE>(f, L, t0, a, B, p0, kt, day);
+        }
+        if (pre1) {
+            if (!(t1.w & TASK_PAIR)) resident_neighbour(f, L, a, t1, p0, pc);
+            task_run<R, VAR, RARE>(f, L, t1, a, B, p0, kt, day);
+        }
+        // Wavefronts with chunks reading other tiles (or entries past the
+        // first two positions) wait, on their own, until the neighbouring
+        // tiles have finished step s-1: their states of step s are then in
+        // `a`, and they no longer read `b` (their step s-1 input), which
+        // this tile overwrites after the barrier below.  A tile without such
+        // chunks lets wavefront 0 make the same wait for `b`.
+        bool ok = true;
+        const bool waits = (i >> 6) == 0 ? wait_any : __builtin_amdgcn_ballot_w64(post0 || post1 || extra) != 0;
+        if (s > 0 && waits) ok = wave_wait_tiles(f, t, pa.flag0 + s, pa, i & 63);
+        if (ok) {
+            if (post0) {
+                resident_neighbour(f, L, a, t0, p0, pc);
+                task_run<R, VAR, RARE>(f, L, t0, a, B, p0, kt, day);
+            }
+            if (post1) {
+                resident_neighbour(f, L, a, t1, p0, pc);
+                task_run<R, VAR, RARE>(f, L, t1, a, B, p0, kt, day);
+            }
+            for (int k = k0 + i + 2 * B; k < kn; k += B) {
+                TaskIn<R> tk = task_load<R>(f, k);
+                resident_neighbour(f, L, a, tk, p0, pc);
+                if (tk.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, tk, a, B, p0, kt, day);
+            }
+        }
+        // every task of the step has run: `a` is no longer read
+        if (!__syncthreads_and(ok)) return;
+        if (live) {
+            const State<R> nw = tile_finish<R, VAR, RARE, false>(f, L, B, i, p, L.st[i], zbp, area, fnc, pad, sd, 0);
+            // other tiles read only the halo cells; every cell at the end
+            if ((pad & kCellHalo) || s == pa.nsteps - 1) st_state<true>(b, p, C, nw);
+            L.st[i] = nw;
+        }
+        // The wavefronts holding the halo cells publish the step as soon as
+        // their stores are complete: each counts itself in, and the last one
+        // sets the flag (the tile's other cells are read by no other tile).
+        if ((i >> 6) < npub) {
+            __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are complete
+            if ((i & 63) == 0 &&
+                __hip_atomic_fetch_add(&lds_pub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) + 1 ==
+                    (unsigned)(s + 1) * (unsigned)npub)
+                __hip_atomic_store(f.tile_flag + t, pa.flag0 + s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();   // LDS states and terms of the step are final
+        if (npub == 0 && threadIdx.x == 0)
+            __hip_atomic_store(f.tile_flag + t, pa.flag0 + s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <class R, int VAR, bool RARE>
+__global__ void __launch_bounds__(kTileMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_resident_tiles(FusedView<R> f, PersistentArgs pa, int remap) {
+    resident_tiles<R, VAR, RARE>(f, pa, remap);
+}
+
 template <class R, int VAR, bool RARE, int SCHEME>
-__global__ void __launch_bounds__(kFusedMaxBlock) k_persistent(FusedView<R> f, PersistentArgs pa) {
+__global__ void __launch_bounds__(SCHEME == 0 ? kTileMaxBlock : kFusedMaxBlock) k_persistent(FusedView<R> f,
+                                                                                           PersistentArgs pa) {
     const int ntiles = (f.C + blockDim.x - 1) / blockDim.x;
     const int64_t g0 = (int64_t)pa.day0 * (pa.spd - 1) + (pa.kt0 - 1);
     const State<R> *a = f.Sin;
@@ -796,6 +1015,24 @@ template <class R> size_t step_lds(int scheme, int block) {
     return scheme == 0 ? tile_lds_bytes(block, (int)sizeof(R)) : 0;
 }
 
+// A launch with more than 64 KiB of dynamic LDS (TILES workgroups of more
+// than 448 threads in fp64) must be allowed per kernel first; done once per
+// kernel.
+hipError_t allow_lds(const void *k, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    static std::mutex mu;
+    static std::vector<std::pair<const void *, size_t>> done;
+    std::lock_guard<std::mutex> g(mu);
+    for (const auto &d : done)
+        if (d.first == k && d.second >= bytes) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) done.push_back({k, bytes});
+    return e;
+}
+
+// Largest workgroup of a scheme's single-mesh kernels (their launch bounds).
+inline int max_block(int scheme) { return scheme == 0 ? kTileMaxBlock : kFusedMaxBlock; }
+
 // Kernel pointer of a (rare, scheme) instantiation.
 #define HYDRO_PICK(KERNEL, R, VAR, rare, scheme)                                                  \
     ((rare) ? ((scheme) == 0 ? (const void *)KERNEL<R, VAR, true, 0> : (const void *)KERNEL<R, VAR, true, 1>) \
@@ -804,12 +1041,14 @@ template <class R> size_t step_lds(int scheme, int block) {
 template <class R, int VAR>
 hipError_t launch_fused_step(const FusedView<R> &f, int kt, int jt, int block, int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
-    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;
+    if (block > max_block(f.scheme) || block != f.block) return hipErrorInvalidValue;
     FusedView<R> a = f;
     void *args[] = {&a, &kt, &jt, &remap};
     const void *k = (f.occ4 && f.scheme == 0)
                         ? (f.rare ? (const void *)k_fused_step_o4<R, VAR, true> : (const void *)k_fused_step_o4<R, VAR, false>)
                         : HYDRO_PICK(k_fused_step, R, VAR, f.rare, f.scheme);
+    const hipError_t e = allow_lds(k, step_lds<R>(f.scheme, block));
+    if (e != hipSuccess) return e;
     return hipLaunchKernel(k, dim3(f.ntiles), dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
@@ -817,9 +1056,11 @@ template <class R, int VAR>
 hipError_t launch_fused_step_scen(const FusedView<R> &f, int nscen, int kt, int jt, int block, int remap,
                                   hipStream_t s) {
     if (f.C == 0 || nscen <= 0) return hipSuccess;
-    if (block > kFusedMaxBlock || block != f.block || f.scheme != 0 || f.rare) return hipErrorInvalidValue;
+    if (block > kTileMaxBlock || block != f.block || f.scheme != 0 || f.rare) return hipErrorInvalidValue;
     FusedView<R> a = f;
     void *args[] = {&a, &kt, &jt, &remap};
+    const hipError_t e = allow_lds((const void *)k_fused_step_scen<R, VAR, false>, step_lds<R>(f.scheme, block));
+    if (e != hipSuccess) return e;
     return hipLaunchKernel((const void *)k_fused_step_scen<R, VAR, false>, dim3(f.ntiles * nscen), dim3(block), args,
                            step_lds<R>(f.scheme, block), s);
 }
@@ -828,13 +1069,15 @@ template <class R, int VAR>
 hipError_t launch_fused_step_tick(const FusedView<R> &f, const StepClock *clock, int offset, int spd, int block,
                                   int remap, hipStream_t s) {
     if (f.C == 0) return hipSuccess;
-    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;
+    if (block > max_block(f.scheme) || block != f.block) return hipErrorInvalidValue;
     FusedView<R> a = f;
     void *args[] = {&a, &clock, &offset, &spd, &remap};
     const void *k = (f.occ4 && f.scheme == 0)
                         ? (f.rare ? (const void *)k_fused_step_tick_o4<R, VAR, true>
                                   : (const void *)k_fused_step_tick_o4<R, VAR, false>)
                         : HYDRO_PICK(k_fused_step_tick, R, VAR, f.rare, f.scheme);
+    const hipError_t e = allow_lds(k, step_lds<R>(f.scheme, block));
+    if (e != hipSuccess) return e;
     return hipLaunchKernel(k, dim3(f.ntiles), dim3(block), args, step_lds<R>(f.scheme, block), s);
 }
 
@@ -879,6 +1122,8 @@ hipError_t persistent_max_grid(int block, bool rare, int scheme, bool occ4, int 
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
+    e = allow_lds(persistent_kernel<R, VAR>(rare, scheme, occ4), step_lds<R>(scheme, block));
+    if (e != hipSuccess) return e;
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel<R, VAR>(rare, scheme, occ4), block,
                                                      step_lds<R>(scheme, block));
     if (e != hipSuccess) return e;
@@ -888,13 +1133,48 @@ hipError_t persistent_max_grid(int block, bool rare, int scheme, bool occ4, int 
 
 template <class R, int VAR>
 hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, int block, int grid, hipStream_t s) {
-    if (block > kFusedMaxBlock || block != f.block) return hipErrorInvalidValue;
+    if (block > max_block(f.scheme) || block != f.block) return hipErrorInvalidValue;
     FusedView<R> a = f;
     PersistentArgs p = pa;
     void *args[] = {&a, &p};
+    const hipError_t e = allow_lds(persistent_kernel<R, VAR>(f.rare != 0, f.scheme, f.occ4 != 0),
+                                   step_lds<R>(f.scheme, block));
+    if (e != hipSuccess) return e;
     return hipLaunchCooperativeKernel(persistent_kernel<R, VAR>(f.rare != 0, f.scheme, f.occ4 != 0), dim3(grid),
                                       dim3(block), args,
                                       step_lds<R>(f.scheme, block), s);
+}
+
+template <class R, int VAR> const void *resident_kernel(bool rare) {
+    return rare ? (const void *)k_resident_tiles<R, VAR, true> : (const void *)k_resident_tiles<R, VAR, false>;
+}
+
+template <class R, int VAR>
+hipError_t resident_capacity(int block, bool rare, int *tiles) {
+    int dev = 0, per_cu = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = allow_lds(resident_kernel<R, VAR>(rare), step_lds<R>(0, block));
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, resident_kernel<R, VAR>(rare), block,
+                                                     step_lds<R>(0, block));
+    if (e != hipSuccess) return e;
+    *tiles = per_cu * cus;
+    return hipSuccess;
+}
+
+template <class R, int VAR>
+hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int remap, hipStream_t s) {
+    if (f.block > kTileMaxBlock || f.scheme != 0 || f.ntiles < 1) return hipErrorInvalidValue;
+    FusedView<R> a = f;
+    PersistentArgs p = pa;
+    void *args[] = {&a, &p, &remap};
+    const hipError_t e = allow_lds(resident_kernel<R, VAR>(f.rare != 0), step_lds<R>(0, f.block));
+    if (e != hipSuccess) return e;
+    return hipLaunchCooperativeKernel(resident_kernel<R, VAR>(f.rare != 0), dim3(f.ntiles), dim3(f.block), args,
+                                      step_lds<R>(0, f.block), s);
 }
 
 template <class R>
@@ -976,6 +1256,8 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
     template hipError_t launch_persistent<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, int, \
                                                   hipStream_t);                                         \
     template hipError_t persistent_max_grid<R, VAR>(int, bool, int, bool, int *);                                     \
+    template hipError_t resident_capacity<R, VAR>(int, bool, int *);                                   \
+    template hipError_t launch_resident<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, hipStream_t); \
     template hipError_t launch_batch_step<R, VAR>(const BatchView<R> &, int, int, int, hipStream_t);   \
     template hipError_t launch_batch_step_tick<R, VAR>(const BatchView<R> &, const int64_t *, int, int, int, \
                                                        hipStream_t);                                    \

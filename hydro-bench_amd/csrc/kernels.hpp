@@ -63,7 +63,9 @@ template <class R> struct FusedView {
     // task_n[t]: 64-entry positions tile t uses (the rest hold TASK_NONE and
     // are not read).  The arrays hold 2*block TASK_NONE entries past the last
     // tile.
-    // cell_pad[p]: bit j set when slot j of cell p is a triangle's padding slot.
+    // cell_pad[p]: bit j (0-3) set when slot j of cell p is a triangle's
+    // padding slot; bit 4 (kCellHalo) when a one-slot task of another tile
+    // reads cell p's state.
     int task_stride;
     const uint32_t *task_w;
     const int32_t *task_nb;
@@ -79,6 +81,10 @@ template <class R> struct FusedView {
     // the launch that tile t has completed
     const int32_t *tile_nb_off, *tile_nb;
     int32_t *tile_flag;
+    // resident kernel: the first tile_pub[t] wavefronts of tile t hold all its
+    // halo cells (kCellHalo); once they have stored them the tile's flag is
+    // published
+    const int32_t *tile_pub;
     // gauge read (gauge_out null when off): the thread of internal cell p
     // writes Z of the step's INPUT state to gauge_out[g] for every gauge g on
     // p (gauge_map[p] the first, -1 none; gauge_next[g] the next gauge on the
@@ -88,16 +94,25 @@ template <class R> struct FusedView {
     double *gauge_out;
 };
 
-// Packed task word of the TILES scheme: bits 0-7 local cell a, 8-9 slot of
-// a, 10-17 local cell b, 18-19 slot of b, 20 pair (both slots of an edge
-// inside the tile: a primary, b mirror), 21-24 slot kind of a one-slot task
-// at preparation (KLAS 7 slots are re-read: they may open).
+// Packed task word of the TILES scheme: bits 0-9 local cell a, 10-11 slot of
+// a, 12-21 local cell b, 22-23 slot of b, 24 pair (both slots of an edge
+// inside the tile: a primary, b mirror), 25-28 slot kind of a one-slot task
+// at preparation (KLAS 7 slots are re-read: they may open), 29 halo (a
+// one-slot task whose neighbour is a cell of another tile).  Tiles hold up
+// to 1024 cells.
+constexpr uint32_t kCellHalo = 0x10;
 constexpr uint32_t TASK_NONE = 0xffffffffu;
-constexpr uint32_t TASK_PAIR = 1u << 20;
+constexpr uint32_t TASK_PAIR = 1u << 24;
+constexpr uint32_t TASK_HALO = 1u << 29;   // one-slot task whose neighbour lies in another tile
 __host__ __device__ inline uint32_t task_word(int la, int ja, int lb, int jb, bool pair, int kind) {
-    return (uint32_t)la | ((uint32_t)ja << 8) | ((uint32_t)lb << 10) | ((uint32_t)jb << 18) |
-           (pair ? TASK_PAIR : 0u) | ((uint32_t)kind << 21);
+    return (uint32_t)la | ((uint32_t)ja << 10) | ((uint32_t)lb << 12) | ((uint32_t)jb << 22) |
+           (pair ? TASK_PAIR : 0u) | ((uint32_t)kind << 25);
 }
+__host__ __device__ inline int task_la(uint32_t w) { return (int)(w & 0x3ff); }
+__host__ __device__ inline int task_ja(uint32_t w) { return (int)((w >> 10) & 3); }
+__host__ __device__ inline int task_lb(uint32_t w) { return (int)((w >> 12) & 0x3ff); }
+__host__ __device__ inline int task_jb(uint32_t w) { return (int)((w >> 22) & 3); }
+__host__ __device__ inline int task_kind(uint32_t w) { return (int)((w >> 25) & 0xf); }
 
 // Time of the next step on a device clock (graph replays advance it):
 // day and kt as the step loop counts them, kt in 1..spd-1.
@@ -106,7 +121,11 @@ struct StepClock {
 };
 
 constexpr int kMaxBlock = 1024;
-constexpr int kFusedMaxBlock = 256;   // fp64 fused step needs > 128 VGPRs
+// Workgroup size limits: the TILES-scheme kernels of one mesh run up to 1024
+// threads (one tile per CU, 128 VGPRs); the CELL scheme (one thread per cell,
+// more than 128 VGPRs in fp64) and the batch kernels up to 256.
+constexpr int kTileMaxBlock = 1024;
+constexpr int kFusedMaxBlock = 256;
 
 // Dynamic LDS of a TILES-scheme workgroup of `block` threads: cell states,
 // beds, ZB1 and the three flux-sum contributions of every slot.
@@ -139,12 +158,19 @@ struct PersistentArgs {
     unsigned *barrier;   // zeroed before launch
     int *error;          // set to 1 if a barrier wait exceeded its spin limit
     unsigned spin_limit;
+    int flag0;           // resident kernel: every tile flag holds flag0 at launch; step s publishes flag0 + s + 1
 };
 // f reads f.Sin on even steps and f.Sout on odd steps.
 template <class R, int VAR>
 hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, int block, int grid, hipStream_t s);
 template <class R, int VAR>
 hipError_t persistent_max_grid(int block, bool rare, int scheme, bool occ4, int *grid);
+// Resident TILES kernel: one workgroup per tile (grid = f.ntiles) for all
+// pa.nsteps steps; capacity = tiles the GPU holds at once.
+template <class R, int VAR>
+hipError_t resident_capacity(int block, bool rare, int *tiles);
+template <class R, int VAR>
+hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int remap, hipStream_t s);
 
 // external H U V Z -> internal records (new order)
 template <class R>

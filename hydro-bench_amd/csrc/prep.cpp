@@ -263,4 +263,29 @@ template void precalculate<float>(int, int, const int32_t *, float *, float *, c
                                   const float *, float, float *, float *, float *, float *, float *,
                                   float *, float *, float *, float *);
 
+// Within each tile of B consecutive positions of `perm`, the cells that
+// share an edge with another tile (a neighbour there, or listed as a
+// neighbour by a cell there) first, the others after them, each group in its
+// previous order.  Tile membership does not change.
+void edge_cells_first(int C, const int32_t *NAC, int B, std::vector<int32_t> &perm) {
+    if (B <= 0) return;
+    std::vector<int32_t> inv(C);
+    for (int p = 0; p < C; p++) inv[perm[p]] = p;
+    std::vector<uint8_t> out(C, 0);   // by original id
+    for (int o = 0; o < C; o++)
+        for (int j = 0; j < 4; j++) {
+            const int n = NAC[4 * (size_t)o + j] - 1;
+            if (n >= 0 && inv[n] / B != inv[o] / B) out[o] = out[n] = 1;
+        }
+    std::vector<int32_t> edge, inner;
+    for (int p0 = 0; p0 < C; p0 += B) {
+        const int pend = std::min(C, p0 + B);
+        edge.clear();
+        inner.clear();
+        for (int p = p0; p < pend; p++) (out[perm[p]] ? edge : inner).push_back(perm[p]);
+        std::copy(edge.begin(), edge.end(), perm.begin() + p0);
+        std::copy(inner.begin(), inner.end(), perm.begin() + p0 + (int)edge.size());
+    }
+}
+
 }  // namespace hydro

@@ -19,6 +19,9 @@ std::vector<int32_t> rcm_order(int C, const int32_t *NAC);
 // connected cells continues from such a seed, so every tile but the last
 // has exactly B cells.  perm[new] = old.
 std::vector<int32_t> tile_order(int C, const int32_t *NAC, int B, const std::vector<int32_t> &along);
+// Within each tile of B consecutive positions, cells with a neighbour in
+// another tile first (their updates finish, and are published, first).
+void edge_cells_first(int C, const int32_t *NAC, int B, std::vector<int32_t> &perm);
 
 // Hilbert-curve order of cell centroids (xc, yc), perm[new] = old.
 std::vector<int32_t> hilbert_order(int C, const double *xc, const double *yc);

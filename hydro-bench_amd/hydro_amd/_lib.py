@@ -38,6 +38,7 @@ DIAG_FIELDS = ("mass", "momentum_x", "momentum_y", "kinetic_energy", "potential_
 # hydro_reorder
 REORDER_NONE, REORDER_RCM, REORDER_HILBERT, REORDER_TILES = range(4)
 SCHEME_TILES, SCHEME_CELL = range(2)
+TILE_BALANCED = -1   # hydro_tune.tile_cells: as many tiles as the GPU holds workgroups at once
 
 
 class HydroError(RuntimeError):
@@ -73,6 +74,7 @@ class Tune(ctypes.Structure):
         ("block", c_int32), ("reorder", c_int32), ("xcd_remap", c_int32), ("graph_chunk", c_int32),
         ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("waves_per_simd", c_int32),
         ("scheme", c_int32), ("tile_cells", c_int32), ("spin_limit", c_int32),
+        ("persistent_loop", c_int32),
     ]
 
 
@@ -103,7 +105,7 @@ def autotune_candidates(hilbert=False):
 
 def default_tune(**kw) -> Tune:
     t = Tune(block=256, reorder=REORDER_TILES, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
-             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0, spin_limit=0)
+             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0, spin_limit=0, persistent_loop=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t

@@ -151,8 +151,11 @@ typedef enum hydro_reorder {
  * Both give the same values bit for bit. */
 typedef enum hydro_scheme { HYDRO_SCHEME_TILES = 0, HYDRO_SCHEME_CELL = 1 } hydro_scheme;
 
+#define HYDRO_TILE_BALANCED (-1)
+
 typedef struct hydro_tune {
-    int32_t block;          /* threads per workgroup of the fused step (64..256, multiple of 64) */
+    int32_t block;          /* threads per workgroup of the fused step (multiple of 64; 64..1024 for
+                               the TILES scheme, 64..256 for the CELL scheme and for batches) */
     int32_t reorder;        /* hydro_reorder */
     int32_t xcd_remap;      /* 1: map workgroups to XCDs in contiguous tile ranges */
     int32_t graph_chunk;    /* steps per captured hipGraph (GRAPH modes) */
@@ -163,10 +166,18 @@ typedef struct hydro_tune {
     int32_t scheme;         /* hydro_scheme */
     int32_t tile_cells;     /* TILES scheme: cells per tile (64..block); 0 = block, or for a mesh
                                with fewer tiles than CUs one tile per CU (at least 64 cells);
-                               batches use block.  With `block` threads per tile this sets the
-                               edges per thread (hydro_plan_stats) */
-    int32_t spin_limit;     /* PERSISTENT modes: sleeps a wait between steps may take before the
-                               launch gives up and reports HYDRO_ERR_HIP (0 = 2^24) */
+                               HYDRO_TILE_BALANCED (-1) = as many tiles as the GPU holds
+                               workgroups of `block` threads at once (1024/block per CU), so
+                               every CU gets the same number; batches use block.  With `block`
+                               threads per tile this sets the edges per thread
+                               (hydro_plan_stats) */
+    int32_t spin_limit;     /* PERSISTENT modes: polls (grid barrier: sleeps) a wait between steps
+                               may take before the launch gives up and reports HYDRO_ERR_HIP
+                               (0 = 2^24) */
+    int32_t persistent_loop; /* PERSISTENT mode, TILES scheme: 0 = one resident workgroup per tile
+                               for the whole launch (tile statics and task entries held on the CU
+                               across steps) whenever every tile fits on the GPU at once, else the
+                               loop below; 1 = workgroups loop over tiles every step */
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */

@@ -238,12 +238,15 @@ def test_determinism_persistent():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
-@pytest.mark.parametrize("tune", [dict(), dict(waves_per_simd=4), dict(persistent_wgs_per_cu=1)],
-                         ids=["default", "occ4", "one-wg-per-cu"])
+@pytest.mark.parametrize("tune", [dict(), dict(block=1024, tile_cells=-1), dict(block=512, tile_cells=-1),
+                                  dict(waves_per_simd=4), dict(persistent_wgs_per_cu=1), dict(persistent_loop=1)],
+                         ids=["resident", "resident-cu-tiles", "resident-512", "occ4", "one-wg-per-cu", "loop"])
 def test_persistent_tiles_equal_fused_207k(dtype, tune):
-    """The persistent TILES kernel (tiles wait only on the tiles they share an
-    edge with) gives the fused launches' bits at full size after 300 steps,
-    also when every workgroup walks several tiles per step."""
+    """The persistent TILES kernels (tiles wait only on the tiles they share an
+    edge with) give the fused launches' bits at full size after 300 steps:
+    the resident kernel (one workgroup per tile for the whole launch, the
+    default), and the loop kernel, also when every workgroup walks several
+    tiles per step."""
     host = cached_mesh("floodplain_207234", dtype=dtype)
     res = {}
     for mode in ("fused", "persistent"):
