@@ -450,18 +450,11 @@ template <class R> int prepare_t(hydro_mesh *m) {
             one.insert(one.end(), sgl_h.begin(), sgl_h.end());
             one.insert(one.end(), bnd_h.begin(), bnd_h.end());
             nsingles += (int)(nlocal + nhalo);
-            // Chunks, in placement order: the tile's own chunks with an
-            // iterative boundary handler (the longest), pairs, its other own
-            // one-slot chunks, and last the chunks reading other tiles (the
-            // resident kernel runs every chunk that reads no other tile
-            // without waiting for the neighbouring tiles, and the others
-            // after it).  Position 0 of every wavefront takes the first
-            // chunks, so each long chunk gets a wavefront of its own; the
-            // others fill position 1, 2, ... of the remaining wavefronts in
-            // an order that visits the four SIMD groups in turn (wavefronts
-            // w, w+4, w+8, w+12 share a SIMD), group 0 last, as it holds the
-            // wavefront with the tile's last, partial row of cells (or one
-            // more cell row than the others).
+            // Chunk classes: the tile's own chunks with an iterative
+            // boundary handler (the longest), pairs, its other own one-slot
+            // chunks, and the chunks reading other tiles (the resident
+            // kernel runs those after waiting for the neighbouring tiles;
+            // every other chunk needs no wait).  Placement below.
             struct Chunk {
                 const std::vector<uint32_t> *src;
                 size_t at, n;
@@ -489,37 +482,50 @@ template <class R> int prepare_t(hydro_mesh *m) {
                 if (L.size() < (size_t)(pos + 1) * kWave) L.resize((size_t)(pos + 1) * kWave, TASK_NONE);
                 std::copy_n(c.src->begin() + c.at, c.n, L.begin() + (size_t)pos * kWave);
             };
+            // Chunks reading other tiles first, at position 0 of the
+            // wavefronts without cells (the last ones; the last wavefront
+            // when every wavefront holds cells), which hold nothing else:
+            // they wait for the neighbouring tiles as the step starts and run
+            // while the others run the tile's own chunks.
+            const int cellw = (pend - p0 + (int)kWave - 1) / (int)kWave;
+            size_t nh = 0;
+            while (nh < ch.size() && ch[ch.size() - 1 - nh].cls == 3) nh++;
+            std::vector<int> hw;
+            if (nh > 0) {
+                for (int w = nwave - 1; w >= cellw; w--) hw.push_back(w);
+                if (hw.empty() || nwave == 1) hw.assign(1, nwave - 1);
+                if ((int)hw.size() >= nwave) hw.resize(nwave - 1);   // keep one wavefront for the rest
+                if (hw.empty()) hw.push_back(0);
+            }
+            std::vector<char> alone(nwave, 0);   // wavefronts holding one long chunk, or the halo chunks
+            for (size_t k = 0; k < nh; k++) {
+                const int w = hw[k % hw.size()];
+                put((int)(k / hw.size()) * nwave + w, ch[ch.size() - nh + k]);
+            }
+            for (int w : hw) alone[w] = (nwave > 1);
+            // the tile's own chunks: position 0 of the other wavefronts (a
+            // long chunk keeps its wavefront to itself), then position 1, 2,
+            // ... in an order that visits the four SIMD groups in turn
+            // (wavefronts w, w+4, w+8, w+12 share a SIMD), group 0 last, as
+            // it holds the wavefront with the tile's last, partial row of
+            // cells (or one more cell row than the others)
+            const size_t nown = ch.size() - nh;
             size_t next = 0;
-            std::vector<char> alone(nwave, 0);   // wavefronts running one long chunk and nothing else
-            for (int w = 0; w < nwave && next < ch.size(); w++) {
-                alone[w] = ch[next].cls == 0 && w + 1 < nwave;
+            for (int w = 0; w < nwave && next < nown; w++) {
+                if (alone[w]) continue;
+                alone[w] = ch[next].cls == 0;
                 put(w, ch[next++]);
             }
             std::vector<int> order;   // wavefront order of positions 1, 2, ...
             for (int base = 0; base < nwave; base += 4)
                 for (int g : {1, 2, 3, 0})
                     if (base + g < nwave && !alone[base + g]) order.push_back(base + g);
-            if (order.empty()) order.push_back(nwave - 1);
-            // chunks reading other tiles go to wavefronts without cells of
-            // their own first (their update pass is the shortest), then to
-            // the first wavefronts
-            const int cellw = (pend - p0 + (int)kWave - 1) / (int)kWave;
-            std::vector<int> horder;
-            for (int w = nwave - 1; w >= cellw; w--)
-                if (!alone[w]) horder.push_back(w);
-            for (int w = 0; w < cellw; w++)
-                if (!alone[w]) horder.push_back(w);
-            if (horder.empty()) horder = order;
-            std::vector<int> hnext(nwave, 1);   // next free position per wavefront
-            for (int round = 1; next < ch.size() && ch[next].cls < 3; round++)
-                for (size_t k = 0; k < order.size() && next < ch.size() && ch[next].cls < 3; k++) {
-                    put(round * nwave + order[k], ch[next++]);
-                    hnext[order[k]] = round + 1;
-                }
-            for (size_t k = 0; next < ch.size(); k++) {
-                const int w = horder[k % horder.size()];
-                put(hnext[w]++ * nwave + w, ch[next++]);
-            }
+            if (order.empty())
+                for (int w = 0; w < nwave; w++)
+                    if (std::find(hw.begin(), hw.end(), w) == hw.end()) order.push_back(w);
+            if (order.empty()) order.push_back(0);
+            for (int round = 1; next < nown; round++)
+                for (size_t k = 0; k < order.size() && next < nown; k++) put(round * nwave + order[k], ch[next++]);
             used_n[t] = used;
             most = std::max(most, L.size());
         }

@@ -797,22 +797,27 @@ __device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const Pers
         time_of(g0 + s, pa.spd, &day, &kt);
         const State<R> *a = (s & 1) ? f.Sout : f.Sin;
         State<R> *b = (s & 1) ? const_cast<State<R> *>(f.Sin) : f.Sout;
-        if (pre0) {
-            if (!(t0.w & TASK_PAIR)) resident_neighbour(f, L, a, t0, p0, pc);
-            task_run<R, VAR, RARE>(f, L, t0, a, B, p0, kt, day);
-        }
-        if (pre1) {
-            if (!(t1.w & TASK_PAIR)) resident_neighbour(f, L, a, t1, p0, pc);
-            task_run<R, VAR, RARE>(f, L, t1, a, B, p0, kt, day);
-        }
         // Wavefronts with chunks reading other tiles (or entries past the
         // first two positions) wait, on their own, until the neighbouring
         // tiles have finished step s-1: their states of step s are then in
         // `a`, and they no longer read `b` (their step s-1 input), which
-        // this tile overwrites after the barrier below.  A tile without such
-        // chunks lets wavefront 0 make the same wait for `b`.
+        // this tile overwrites after the barrier below.  They wait as the
+        // step starts (the plan gives them those chunks only), while the
+        // others run the tile's own chunks.  A tile without such chunks lets
+        // wavefront 0 make the same wait for `b`, after its own chunks.
+        const bool halo_wave = __builtin_amdgcn_ballot_w64(post0 || post1 || extra) != 0;
+        const bool waits = (i >> 6) == 0 ? wait_any : halo_wave;
+        if (!halo_wave) {
+            if (pre0) {
+                if (!(t0.w & TASK_PAIR)) resident_neighbour(f, L, a, t0, p0, pc);
+                task_run<R, VAR, RARE>(f, L, t0, a, B, p0, kt, day);
+            }
+            if (pre1) {
+                if (!(t1.w & TASK_PAIR)) resident_neighbour(f, L, a, t1, p0, pc);
+                task_run<R, VAR, RARE>(f, L, t1, a, B, p0, kt, day);
+            }
+        }
         bool ok = true;
-        const bool waits = (i >> 6) == 0 ? wait_any : __builtin_amdgcn_ballot_w64(post0 || post1 || extra) != 0;
         if (s > 0 && waits) ok = wave_wait_tiles(f, t, pa.flag0 + s, pa, i & 63);
         if (ok) {
             if (post0) {
@@ -827,6 +832,16 @@ __device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const Pers
                 TaskIn<R> tk = task_load<R>(f, k);
                 resident_neighbour(f, L, a, tk, p0, pc);
                 if (tk.w != TASK_NONE) task_run<R, VAR, RARE>(f, L, tk, a, B, p0, kt, day);
+            }
+            if (halo_wave) {   // own chunks of a wavefront that also reads other tiles
+                if (pre0) {
+                    if (!(t0.w & TASK_PAIR)) resident_neighbour(f, L, a, t0, p0, pc);
+                    task_run<R, VAR, RARE>(f, L, t0, a, B, p0, kt, day);
+                }
+                if (pre1) {
+                    if (!(t1.w & TASK_PAIR)) resident_neighbour(f, L, a, t1, p0, pc);
+                    task_run<R, VAR, RARE>(f, L, t1, a, B, p0, kt, day);
+                }
             }
         }
         // every task of the step has run: `a` is no longer read
