@@ -593,9 +593,9 @@ template <class R> int prepare_t(hydro_mesh *m) {
         for (int t = 0; t < nt; t++) m->max_tile_nb = std::max(m->max_tile_nb, off[t + 1] - off[t]);
         if (lst.empty()) lst.push_back(0);
         if ((rc = h2d(m->tile_nb_off, off)) || (rc = h2d(m->tile_nb, lst)) ||
-            (rc = m->tile_flag.alloc((size_t)nt * sizeof(int32_t))))
+            (rc = m->tile_flag.alloc((size_t)nt * kFlagStride * sizeof(int32_t))))
             return rc;
-        HIP_TRY(hipMemset(m->tile_flag.p, 0, (size_t)nt * sizeof(int32_t)));
+        HIP_TRY(hipMemset(m->tile_flag.p, 0, m->tile_flag.bytes));
         m->flag_base = 0;
     }
     if ((rc = h2d(m->ZBC, zbc_n)) || (rc = h2d(m->AREA, area_n)) || (rc = h2d(m->FNC, fnc_n)) ||
@@ -852,7 +852,7 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
         // the flags need no reset: this launch counts on from where the
         // last one left them (restarting from 0 long before int32 overflow)
         if ((int64_t)m->flag_base + n > (1 << 30)) {
-            HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles, s));
+            HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles * kFlagStride, s));
             m->flag_base = 0;
         }
         pa.flag0 = m->flag_base;
@@ -871,7 +871,7 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
         grid = std::max(1, std::min(grid, ntiles));
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pa.barrier);
         HIP_TRY(hipGetLastError());
-        if (f.scheme == HYDRO_SCHEME_TILES) HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles, s));
+        if (f.scheme == HYDRO_SCHEME_TILES) HIP_TRY(launch_fill_i32(m->tile_flag.as<int32_t>(), 0, ntiles * kFlagStride, s));
         HIP_TRY((launch_persistent<R, VAR>(f, pa, block, grid, s)));
         m->flag_base = (f.scheme == HYDRO_SCHEME_TILES) ? n : m->flag_base;
     }

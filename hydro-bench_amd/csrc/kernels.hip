@@ -620,7 +620,7 @@ __device__ __forceinline__ bool wait_tiles(const FusedView<R> &f, int t, int tar
     for (int k = a + (int)threadIdx.x; k < b; k += blockDim.x) {
         const int u = f.tile_nb[k];
         unsigned spins = 0;
-        while (__hip_atomic_load(f.tile_flag + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(f.tile_flag + (size_t)u * kFlagStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if ((++spins & 63) == 0 &&
                 (spins > pa.spin_limit || __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -643,7 +643,7 @@ __device__ __forceinline__ bool wave_wait_tiles(const FusedView<R> &f, int t, in
     for (int k = a + lane; k < b; k += 64) {
         const int u = f.tile_nb[k];
         unsigned spins = 0;
-        while (__hip_atomic_load(f.tile_flag + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(f.tile_flag + (size_t)u * kFlagStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if ((++spins & 63) == 0 &&
                 (spins > pa.spin_limit || __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -672,7 +672,7 @@ __device__ __forceinline__ void persistent_tiles(const FusedView<R> &f, const Pe
                 return;
             __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are complete
             __syncthreads();
-            if (threadIdx.x == 0) __hip_atomic_store(f.tile_flag + t, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (threadIdx.x == 0) __hip_atomic_store(f.tile_flag + (size_t)t * kFlagStride, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -860,11 +860,11 @@ __device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const Pers
             if ((i & 63) == 0 &&
                 __hip_atomic_fetch_add(&lds_pub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) + 1 ==
                     (unsigned)(s + 1) * (unsigned)npub)
-                __hip_atomic_store(f.tile_flag + t, pa.flag0 + s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(f.tile_flag + (size_t)t * kFlagStride, pa.flag0 + s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();   // LDS states and terms of the step are final
         if (npub == 0 && threadIdx.x == 0)
-            __hip_atomic_store(f.tile_flag + t, pa.flag0 + s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(f.tile_flag + (size_t)t * kFlagStride, pa.flag0 + s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -80,7 +80,7 @@ template <class R> struct FusedView {
     // tile_nb[tile_nb_off[t] .. tile_nb_off[t+1]); tile_flag[t] = steps of
     // the launch that tile t has completed
     const int32_t *tile_nb_off, *tile_nb;
-    int32_t *tile_flag;
+    int32_t *tile_flag;        // tile t's flag at tile_flag[t * kFlagStride]
     // resident kernel: the first tile_pub[t] wavefronts of tile t hold all its
     // halo cells (kCellHalo); once they have stored them the tile's flag is
     // published
@@ -101,6 +101,10 @@ template <class R> struct FusedView {
 // one-slot task whose neighbour is a cell of another tile).  Tiles hold up
 // to 1024 cells.
 constexpr uint32_t kCellHalo = 0x10;
+// Tile step flags sit 256 bytes apart: every tile polls its neighbours' flags
+// every step, and flags sharing cache lines would put those polls on a few
+// lines of one memory channel.
+constexpr int kFlagStride = 64;
 constexpr uint32_t TASK_NONE = 0xffffffffu;
 constexpr uint32_t TASK_PAIR = 1u << 24;
 constexpr uint32_t TASK_HALO = 1u << 29;   // one-slot task whose neighbour lies in another tile
