@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--mode", default="persistent",
                     choices=["persistent", "graph", "fused", "two_kernel", "graph_two_kernel"])
     ap.add_argument("--block", type=int, default=1024, help="threads per workgroup of the reordered-layout kernels")
+    ap.add_argument("--launch", default="plain", choices=["plain", "cooperative"],
+                    help="launch of the resident kernel (plain: ~10 us sooner to start; nothing else runs on the GPU)")
     ap.add_argument("--tile", type=int, default=-1,
                     help="cells per tile; -1: balanced, as many tiles as the GPU holds workgroups at once "
                          "(one per CU at --block 1024)")
@@ -95,13 +97,13 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8, "copies": 5}))
 
 
-def time_gpu(host, mode, steps, warmup, block, tile=0):
+def time_gpu(host, mode, steps, warmup, block, tile=0, launch="cooperative"):
     import torch
 
     import hydro_amd
     from hydro_amd.device import DeviceMesh
 
-    tune = hydro_amd.default_tune(block=block, tile_cells=tile)
+    tune = hydro_amd.default_tune(block=block, tile_cells=tile, resident_launch=1 if launch == "plain" else 0)
     dm = DeviceMesh(host, device=0, tune=tune)
     tiles = dm.plan_stats()[0]
     init = host.state()
@@ -457,7 +459,7 @@ def main():
 
     host64 = hydro_amd.make_config(WORKLOAD, dtype=np.float64)
     steps = min(a.steps, host64.max_steps)
-    ms, wall, launches, st, tiles = time_gpu(host64, a.mode, steps, a.warmup, a.block, a.tile)
+    ms, wall, launches, st, tiles = time_gpu(host64, a.mode, steps, a.warmup, a.block, a.tile, a.launch)
     if not np.all(np.isfinite(st["H"])):
         raise SystemExit("non-finite state after the timed steps")
     C = host64.CELL
@@ -496,7 +498,8 @@ def main():
         "data": "synthetic (seeded 207,234-cell floodplain mesh, DT=0.5 s)",
         "config": {"workload": WORKLOAD, "cells": C, "edges": host64.n_edges, "dt": host64.DT,
                    "mode": a.mode, "block": a.block, "tiles": tiles,
-                   "tile_cells": "balanced" if a.tile == -1 else (a.tile or a.block), "variant": "A"},
+                   "tile_cells": "balanced" if a.tile == -1 else (a.tile or a.block), "variant": "A",
+                   "launch": a.launch if resident else None},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell_step": BYTES_PER_CELL_STEP["f64"], "launches": launches,
@@ -505,7 +508,7 @@ def main():
     }
     if not a.no_fp32:
         host32 = hydro_amd.make_config(WORKLOAD, dtype=np.float32)
-        ms32, _, l32, _, _ = time_gpu(host32, a.mode, steps, a.warmup, a.block, a.tile)
+        ms32, _, l32, _, _ = time_gpu(host32, a.mode, steps, a.warmup, a.block, a.tile, a.launch)
         v32 = C * steps / (ms32 / 1e3)
         out["fp32"] = {"value": v32, "ms_per_step": ms32 / steps,
                        "roofline_frac": v32 * BYTES_PER_CELL_STEP["f32"] / 1e9 / HBM_PEAK_GBS}

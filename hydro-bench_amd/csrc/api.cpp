@@ -856,7 +856,7 @@ template <class R, int VAR> int steps_persistent(hydro_mesh *m, int n, hipStream
             m->flag_base = 0;
         }
         pa.flag0 = m->flag_base;
-        HIP_TRY((launch_resident<R, VAR>(f, pa, m->tune.xcd_remap, s)));
+        HIP_TRY((launch_resident<R, VAR>(f, pa, m->tune.xcd_remap, m->tune.resident_launch == 1, s)));
         m->flag_base += n;
     } else {
         int maxg = 0;
@@ -908,6 +908,7 @@ hydro_tune default_tune() {
     t.tile_cells = 0;
     t.spin_limit = 0;
     t.persistent_loop = 0;
+    t.resident_launch = 0;
     return t;
 }
 
@@ -1048,6 +1049,8 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
     if (t.spin_limit < 0) return fail(HYDRO_ERR_INVALID, "spin_limit %d", t.spin_limit);
     if (t.persistent_loop != 0 && t.persistent_loop != 1)
         return fail(HYDRO_ERR_INVALID, "persistent_loop %d (0 or 1)", t.persistent_loop);
+    if (t.resident_launch != 0 && t.resident_launch != 1)
+        return fail(HYDRO_ERR_INVALID, "resident_launch %d (0 or 1)", t.resident_launch);
     int tile = t.block;
     if (t.scheme == HYDRO_SCHEME_TILES && t.tile_cells > 0) {
         tile = t.tile_cells;

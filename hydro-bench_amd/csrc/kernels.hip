@@ -1180,14 +1180,19 @@ hipError_t resident_capacity(int block, bool rare, int *tiles) {
     return hipSuccess;
 }
 
+// plain: an ordinary launch; the caller has checked that every tile fits on
+// the GPU at once and that nothing else holds CUs meanwhile.
 template <class R, int VAR>
-hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int remap, hipStream_t s) {
+hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int remap, bool plain, hipStream_t s) {
     if (f.block > kTileMaxBlock || f.scheme != 0 || f.ntiles < 1) return hipErrorInvalidValue;
     FusedView<R> a = f;
     PersistentArgs p = pa;
     void *args[] = {&a, &p, &remap};
     const hipError_t e = allow_lds(resident_kernel<R, VAR>(f.rare != 0), step_lds<R>(0, f.block));
     if (e != hipSuccess) return e;
+    if (plain)
+        return hipLaunchKernel(resident_kernel<R, VAR>(f.rare != 0), dim3(f.ntiles), dim3(f.block), args,
+                               step_lds<R>(0, f.block), s);
     return hipLaunchCooperativeKernel(resident_kernel<R, VAR>(f.rare != 0), dim3(f.ntiles), dim3(f.block), args,
                                       step_lds<R>(0, f.block), s);
 }
@@ -1272,7 +1277,7 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
                                                   hipStream_t);                                         \
     template hipError_t persistent_max_grid<R, VAR>(int, bool, int, bool, int *);                                     \
     template hipError_t resident_capacity<R, VAR>(int, bool, int *);                                   \
-    template hipError_t launch_resident<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, hipStream_t); \
+    template hipError_t launch_resident<R, VAR>(const FusedView<R> &, const PersistentArgs &, int, bool, hipStream_t); \
     template hipError_t launch_batch_step<R, VAR>(const BatchView<R> &, int, int, int, hipStream_t);   \
     template hipError_t launch_batch_step_tick<R, VAR>(const BatchView<R> &, const int64_t *, int, int, int, \
                                                        hipStream_t);                                    \

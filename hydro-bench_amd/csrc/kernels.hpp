@@ -174,7 +174,7 @@ hipError_t persistent_max_grid(int block, bool rare, int scheme, bool occ4, int 
 template <class R, int VAR>
 hipError_t resident_capacity(int block, bool rare, int *tiles);
 template <class R, int VAR>
-hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int remap, hipStream_t s);
+hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int remap, bool plain, hipStream_t s);
 
 // external H U V Z -> internal records (new order)
 template <class R>

@@ -74,7 +74,7 @@ class Tune(ctypes.Structure):
         ("block", c_int32), ("reorder", c_int32), ("xcd_remap", c_int32), ("graph_chunk", c_int32),
         ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("waves_per_simd", c_int32),
         ("scheme", c_int32), ("tile_cells", c_int32), ("spin_limit", c_int32),
-        ("persistent_loop", c_int32),
+        ("persistent_loop", c_int32), ("resident_launch", c_int32),
     ]
 
 
@@ -105,7 +105,7 @@ def autotune_candidates(hilbert=False):
 
 def default_tune(**kw) -> Tune:
     t = Tune(block=256, reorder=REORDER_TILES, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
-             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0, spin_limit=0, persistent_loop=0)
+             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0, spin_limit=0, persistent_loop=0, resident_launch=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t

@@ -178,6 +178,13 @@ typedef struct hydro_tune {
                                for the whole launch (tile statics and task entries held on the CU
                                across steps) whenever every tile fits on the GPU at once, else the
                                loop below; 1 = workgroups loop over tiles every step */
+    int32_t resident_launch; /* the resident kernel's launch: 0 = cooperative (the runtime
+                               guarantees every workgroup is resident); 1 = plain launch, about
+                               10 us sooner to start: the library has checked that every tile
+                               fits on the GPU at once, and the caller guarantees that no kernel
+                               of another stream or process holds CUs while it runs (two resident
+                               launches sharing the GPU could each wait for the other's CUs until
+                               the spin limit) */
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */

@@ -238,9 +238,11 @@ def test_determinism_persistent():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
-@pytest.mark.parametrize("tune", [dict(), dict(block=1024, tile_cells=-1), dict(block=512, tile_cells=-1),
-                                  dict(waves_per_simd=4), dict(persistent_wgs_per_cu=1), dict(persistent_loop=1)],
-                         ids=["resident", "resident-cu-tiles", "resident-512", "occ4", "one-wg-per-cu", "loop"])
+@pytest.mark.parametrize("tune", [dict(), dict(block=1024, tile_cells=-1), dict(block=1024, tile_cells=-1, resident_launch=1),
+                                  dict(block=512, tile_cells=-1), dict(waves_per_simd=4), dict(persistent_wgs_per_cu=1),
+                                  dict(persistent_loop=1)],
+                         ids=["resident", "resident-cu-tiles", "resident-plain-launch", "resident-512", "occ4",
+                              "one-wg-per-cu", "loop"])
 def test_persistent_tiles_equal_fused_207k(dtype, tune):
     """The persistent TILES kernels (tiles wait only on the tiles they share an
     edge with) give the fused launches' bits at full size after 300 steps:
