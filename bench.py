@@ -482,7 +482,11 @@ def main():
     tpath = os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{WORKLOAD}_f64_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        # a multi-step launch: its fixed part plus the per-step part for
+        # this launch's steps (counter fit over two launch lengths)
+        traffic = (tj["fixed_bytes"] + tj["bytes_per_step"] * steps_per_launch if "bytes_per_step" in tj
+                   else tj.get("bytes_per_launch"))
     out = {
         "metric": "cell-steps/sec",
         "value": value,

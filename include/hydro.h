@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define HYDRO_ABI_VERSION 3
+#define HYDRO_ABI_VERSION 4
 
 typedef enum hydro_status {
     HYDRO_OK = 0,
