@@ -514,6 +514,15 @@ __device__ __forceinline__ void time_of(int64_t g, int spd, int *day, int *kt) {
     *kt = 1 + (int)(g % per);
 }
 
+// The next step's (day, kt) from this step's, as the step loop counts them
+// (kt in 1..spd-1): what time_of(g + 1) gives, without its 64-bit division.
+__device__ __forceinline__ void time_next(int spd, int *day, int *kt) {
+    if (++*kt >= spd) {
+        *kt = 1;
+        ++*day;
+    }
+}
+
 // (day, kt) of the step `k` steps after the clock's (one uniform 32-bit
 // division per wave).
 __device__ __forceinline__ void clock_at(StepClock c, int k, int spd, int *day, int *kt) {
@@ -660,9 +669,9 @@ template <class R, int VAR, bool RARE>
 __device__ __forceinline__ void persistent_tiles(const FusedView<R> &f, const PersistentArgs &pa) {
     const int ntiles = f.ntiles;
     const int64_t g0 = (int64_t)pa.day0 * (pa.spd - 1) + (pa.kt0 - 1);
-    for (int s = 0; s < pa.nsteps; s++) {
-        int day, kt;
-        time_of(g0 + s, pa.spd, &day, &kt);
+    int day, kt;
+    time_of(g0, pa.spd, &day, &kt);
+    for (int s = 0; s < pa.nsteps; s++, time_next(pa.spd, &day, &kt)) {
         const State<R> *a = (s & 1) ? f.Sout : f.Sin;
         State<R> *b = (s & 1) ? const_cast<State<R> *>(f.Sin) : f.Sout;
         for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -724,10 +733,12 @@ template <class R> __device__ __forceinline__ void hold(TaskIn<R> &t) {
 // for this tile's next step and, for the halo cells (and every cell at the
 // last step), to the step's output buffer.  Same arithmetic as tile_cells,
 // so the same bits.
-template <class R, int VAR, bool RARE>
+// BLK: the workgroup size when fixed at compile time (its LDS offsets are
+// then constants), 0 = f.block.
+template <class R, int VAR, bool RARE, int BLK>
 __device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const PersistentArgs &pa, int remap) {
     const int t = tile_of(blockIdx.x, f.ntiles, remap);
-    const int B = f.block, C = f.C;
+    const int B = BLK ? BLK : f.block, C = f.C;
     const int p0 = t * f.tile;
     const TileLds<R> L = tile_lds<R>(B);
     R zbp, area, fnc;
@@ -774,7 +785,9 @@ __device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const Pers
                                                                    (t1.w != TASK_NONE && (t1.w & TASK_HALO)) ||
                                                                    extra) != 0;
     const int64_t g0 = (int64_t)pa.day0 * (pa.spd - 1) + (pa.kt0 - 1);
-    for (int s = 0; s < pa.nsteps; s++) {
+    int day, kt;
+    time_of(g0, pa.spd, &day, &kt);
+    for (int s = 0; s < pa.nsteps; s++, time_next(pa.spd, &day, &kt)) {
         // The thread index and the values held across steps pass through an
         // empty asm every step, so nothing derived from them (LDS offsets,
         // the slots' SIDE*COSF products, ...) is hoisted out of the loop into
@@ -793,8 +806,6 @@ __device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const Pers
         const int p = p0 + i;
         const bool live = i < f.tile && p < C;
         const int pc = live ? p : C - 1;
-        int day, kt;
-        time_of(g0 + s, pa.spd, &day, &kt);
         const State<R> *a = (s & 1) ? f.Sout : f.Sin;
         State<R> *b = (s & 1) ? const_cast<State<R> *>(f.Sin) : f.Sout;
         // Wavefronts with chunks reading other tiles (or entries past the
@@ -868,10 +879,10 @@ __device__ __forceinline__ void resident_tiles(const FusedView<R> &f, const Pers
     }
 }
 
-template <class R, int VAR, bool RARE>
+template <class R, int VAR, bool RARE, int BLK>
 __global__ void __launch_bounds__(kTileMaxBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_resident_tiles(FusedView<R> f, PersistentArgs pa, int remap) {
-    resident_tiles<R, VAR, RARE>(f, pa, remap);
+    resident_tiles<R, VAR, RARE, BLK>(f, pa, remap);
 }
 
 template <class R, int VAR, bool RARE, int SCHEME>
@@ -881,9 +892,9 @@ __global__ void __launch_bounds__(SCHEME == 0 ? kTileMaxBlock : kFusedMaxBlock) 
     const int64_t g0 = (int64_t)pa.day0 * (pa.spd - 1) + (pa.kt0 - 1);
     const State<R> *a = f.Sin;
     State<R> *b = f.Sout;
-    for (int s = 0; s < pa.nsteps; s++) {
-        int day, kt;
-        time_of(g0 + s, pa.spd, &day, &kt);
+    int day, kt;
+    time_of(g0, pa.spd, &day, &kt);
+    for (int s = 0; s < pa.nsteps; s++, time_next(pa.spd, &day, &kt)) {
         for (int t = blockIdx.x; t < ntiles; t += gridDim.x) step_tile<R, VAR, RARE, SCHEME>(f, a, b, t, kt, day);
         State<R> *tmp = const_cast<State<R> *>(a);
         a = b;
@@ -1160,8 +1171,11 @@ hipError_t launch_persistent(const FusedView<R> &f, const PersistentArgs &pa, in
                                       step_lds<R>(f.scheme, block), s);
 }
 
-template <class R, int VAR> const void *resident_kernel(bool rare) {
-    return rare ? (const void *)k_resident_tiles<R, VAR, true> : (const void *)k_resident_tiles<R, VAR, false>;
+// the 1024-thread build for 1024-thread workgroups, else the general one
+template <class R, int VAR> const void *resident_kernel(bool rare, int block) {
+    if (block == 1024)
+        return rare ? (const void *)k_resident_tiles<R, VAR, true, 1024> : (const void *)k_resident_tiles<R, VAR, false, 1024>;
+    return rare ? (const void *)k_resident_tiles<R, VAR, true, 0> : (const void *)k_resident_tiles<R, VAR, false, 0>;
 }
 
 template <class R, int VAR>
@@ -1171,9 +1185,9 @@ hipError_t resident_capacity(int block, bool rare, int *tiles) {
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = allow_lds(resident_kernel<R, VAR>(rare), step_lds<R>(0, block));
+    e = allow_lds(resident_kernel<R, VAR>(rare, block), step_lds<R>(0, block));
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, resident_kernel<R, VAR>(rare), block,
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, resident_kernel<R, VAR>(rare, block), block,
                                                      step_lds<R>(0, block));
     if (e != hipSuccess) return e;
     *tiles = per_cu * cus;
@@ -1188,12 +1202,12 @@ hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int 
     FusedView<R> a = f;
     PersistentArgs p = pa;
     void *args[] = {&a, &p, &remap};
-    const hipError_t e = allow_lds(resident_kernel<R, VAR>(f.rare != 0), step_lds<R>(0, f.block));
+    const hipError_t e = allow_lds(resident_kernel<R, VAR>(f.rare != 0, f.block), step_lds<R>(0, f.block));
     if (e != hipSuccess) return e;
     if (plain)
-        return hipLaunchKernel(resident_kernel<R, VAR>(f.rare != 0), dim3(f.ntiles), dim3(f.block), args,
+        return hipLaunchKernel(resident_kernel<R, VAR>(f.rare != 0, f.block), dim3(f.ntiles), dim3(f.block), args,
                                step_lds<R>(0, f.block), s);
-    return hipLaunchCooperativeKernel(resident_kernel<R, VAR>(f.rare != 0), dim3(f.ntiles), dim3(f.block), args,
+    return hipLaunchCooperativeKernel(resident_kernel<R, VAR>(f.rare != 0, f.block), dim3(f.ntiles), dim3(f.block), args,
                                       step_lds<R>(0, f.block), s);
 }
 
