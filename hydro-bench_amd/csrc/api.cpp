@@ -297,6 +297,17 @@ template <class R> int prepare_t(hydro_mesh *m) {
     m->need_zb1 = !zb1_ok;
     m->need_slcs = !slcs_ok;
 
+    // the state at planning time, in the new order (H, Z, ZB1): the pairs of
+    // a tile are grouped by the wet/dry branch they take in it
+    std::vector<R> Hs, Zs;
+    if ((rc = d2h(Hs, d.H, C)) || (rc = d2h(Zs, d.Z, C))) return rc;
+    std::vector<R> h_n(C), z_n(C), zi_n(C);
+    for (int p = 0; p < C; p++) {
+        const int o = perm[p];
+        h_n[p] = Hs[o];
+        z_n[p] = Zs[o];
+        zi_n[p] = std::max(Zs[o], ZB1[o]);
+    }
     std::vector<R> zbc_n(C), area_n(C), fnc_n(C), zb1_n(m->need_zb1 ? C : 0), dta_n(C);
     std::vector<int32_t> nbr_n(S4), orig_n(C);
     std::vector<uint8_t> meta_n(S4);
@@ -373,6 +384,20 @@ template <class R> int prepare_t(hydro_mesh *m) {
         };
         // slot ids as 4*p + j during planning
         std::vector<uint32_t> pairs, sgl, bnd, sgl_h, bnd_h;
+        Scalars<R> S_plan;
+        S_plan.HM1 = (R)d.HM1;
+        S_plan.HM2 = (R)d.HM2;
+        // stable order of a task list by an integer class of each task
+        std::vector<std::pair<int, uint32_t>> keyed;
+        auto group_by = [&](std::vector<uint32_t> &v, auto cls) {
+            keyed.resize(v.size());
+            for (size_t q = 0; q < v.size(); q++) keyed[q] = {cls(v[q]), v[q]};
+            std::stable_sort(keyed.begin(), keyed.end(),
+                             [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
+                                 return x.first < y.first;
+                             });
+            for (size_t q = 0; q < v.size(); q++) v[q] = keyed[q].second;
+        };
         for (int t = 0; t < ntiles; t++) {
             pairs.clear();
             sgl.clear();
@@ -415,6 +440,28 @@ template <class R> int prepare_t(hydro_mesh *m) {
                     }
                     (halo ? sgl_h : sgl).push_back(task_word(p - p0, j, 0, 0, false, kind) | hb);
                 }
+            // Interior tasks by the wet/dry branches their sides take in the
+            // current state (pairs: both sides wet, one side, neither; one-
+            // slot tasks: wet or not), so most chunks hold one kind: a
+            // wavefront runs the Riemann solve if any lane has a wet side,
+            // and the dry-side ladder code if any lane has a dry one.  The
+            // order of the tasks does not change the results (each slot's
+            // terms have their own place; the sums run in slot order), only
+            // the time; call hydro_prepare again to regroup once the wet area
+            // has moved.
+            auto side_wet = [&](int a, int b) {
+                const R HC = std::max(h_n[b], S_plan.HM1), ZC = std::max(zbc_n[b], z_n[b]);
+                return ladder_branch<R>(S_plan, h_n[a], zi_n[a], zbc_n[a], HC, zbc_n[b], ZC) == 7;
+            };
+            auto pair_class = [&](uint32_t w) {
+                const int a = p0 + (int)task_la(w), b = p0 + (int)task_lb(w);
+                return 2 - (int)side_wet(a, b) - (int)side_wet(b, a);
+            };
+            auto slot_class = [&](uint32_t w) {
+                const int a = p0 + (int)task_la(w), n = nbr_n[(size_t)task_ja(w) * C + a];
+                return (n >= 0 && side_wet(a, n)) ? 0 : 1;
+            };
+            group_by(pairs, pair_class);
             // leftover pairs -> one-slot tasks when the last chunk of the
             // tile's own one-slot tasks has room
             const size_t left = pairs.size() % kWave, ones = sgl.size() + bnd.size();
@@ -430,6 +477,8 @@ template <class R> int prepare_t(hydro_mesh *m) {
                 npairs -= (int)left;
                 sgl.insert(sgl.begin(), split.begin(), split.end());
             }
+            group_by(sgl, slot_class);
+            group_by(sgl_h, slot_class);
             // Boundary slots by cost of their handler (walls and open edges
             // cheap; weirs, breaches; the fixed-point iterations of rating-
             // curve, level and discharge edges), so the costly ones share as

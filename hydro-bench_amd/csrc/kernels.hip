@@ -371,7 +371,6 @@ __device__ __forceinline__ bool tile_cells(const FusedView<R> &f, const State<R>
     if constexpr (!std::is_same<Wait, NoWait>::value) asm volatile("" : "+v"(i));
     const int p0 = t * f.tile, p = p0 + i;
     const bool live = i < f.tile && p < C;
-    const Scalars<R> S = f.S;
     const TileLds<R> L = tile_lds<R>(B);
     // Phase 1: the cell's state and statics, and the tile's first two task
     // entries per thread, all independent loads in flight together.

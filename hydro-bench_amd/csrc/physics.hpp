@@ -372,10 +372,9 @@ __host__ __device__ inline Flux<R> boundary_flux(int kind, const Scalars<R> &S, 
 // outflow), 5/6 HC<=HM2, 7 both wet (Riemann problem, F:149-200).  For
 // branches 0..6 *f receives the flux.  At most one pow(., 1.5) and one sqrt
 // per slot, so their expansions appear once in the code.
+// The branch alone (the task planner groups slots by it).
 template <class R>
-__host__ __device__ __forceinline__ int ladder(const Scalars<R> &S, R H1, R ZI, R ZBp, R HC, R BC, R ZC, R UC,
-                                               R VC, R QL1, R QL2, R COSJ, R SINJ, Flux<R> *f) {
-    using K = Consts<R>;
+__host__ __device__ __forceinline__ int ladder_branch(const Scalars<R> &S, R H1, R ZI, R ZBp, R HC, R BC, R ZC) {
     int br;
     if (H1 <= S.HM1 && HC <= S.HM1) br = 0;
     else if (ZI <= BC) br = 1;
@@ -383,6 +382,14 @@ __host__ __device__ __forceinline__ int ladder(const Scalars<R> &S, R H1, R ZI, 
     else if (H1 <= S.HM2) br = (ZC > ZI) ? 3 : 4;
     else if (HC <= S.HM2) br = (ZI > ZC) ? 5 : 6;
     else br = 7;
+    return br;
+}
+
+template <class R>
+__host__ __device__ __forceinline__ int ladder(const Scalars<R> &S, R H1, R ZI, R ZBp, R HC, R BC, R ZC, R UC,
+                                               R VC, R QL1, R QL2, R COSJ, R SINJ, Flux<R> *f) {
+    using K = Consts<R>;
+    const int br = ladder_branch<R>(S, H1, ZI, ZBp, HC, BC, ZC);
     if (br == 7) return br;
     // Branch sets as bit masks over br (no multi-case switch: see osher):
     // pow(., 1.5) for {1, 2, 4, 6} of HC in {1, 6}, sqrt for {3, 5}.
