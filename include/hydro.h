@@ -197,7 +197,10 @@ int hydro_mesh_destroy(hydro_mesh *m);
 
 /* Build the reordered internal layout used by the FUSED, GRAPH and
  * PERSISTENT modes (host-side reorder + device upload, synchronous).
- * tune may be NULL for defaults. */
+ * tune may be NULL for defaults.  The task plan groups each tile's interior
+ * edges by the wet/dry branch they take in the current state (H, Z); the
+ * results do not depend on it, only the speed: calling it again once the
+ * wet area has moved regroups them. */
 int hydro_prepare(hydro_mesh *m, const hydro_tune *tune);
 
 /* Autotuner (no counterpart in the reference): times each candidate tuning of
