@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--tile", type=int, default=-1,
                     help="cells per tile; -1: balanced, as many tiles as the GPU holds workgroups at once "
                          "(one per CU at --block 1024)")
+    ap.add_argument("--regroup", type=int, default=0,
+                    help="hydro_tune.regroup_steps: steps per launch and between regroupings of the task plan "
+                         "by the wet/dry state (0 = the library's 200, -1 = one launch, no regrouping)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -97,13 +100,14 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8, "copies": 5}))
 
 
-def time_gpu(host, mode, steps, warmup, block, tile=0, launch="cooperative"):
+def time_gpu(host, mode, steps, warmup, block, tile=0, launch="cooperative", regroup=0):
     import torch
 
     import hydro_amd
     from hydro_amd.device import DeviceMesh
 
-    tune = hydro_amd.default_tune(block=block, tile_cells=tile, resident_launch=1 if launch == "plain" else 0)
+    tune = hydro_amd.default_tune(block=block, tile_cells=tile, resident_launch=1 if launch == "plain" else 0,
+                                  regroup_steps=regroup)
     dm = DeviceMesh(host, device=0, tune=tune)
     tiles = dm.plan_stats()[0]
     init = host.state()
@@ -126,6 +130,8 @@ def time_gpu(host, mode, steps, warmup, block, tile=0, launch="cooperative"):
     ms = e0.elapsed_time(e1)
     launches = {"persistent": 1, "graph": steps, "fused": steps, "two_kernel": 2 * steps,
                 "graph_two_kernel": 2 * steps}[mode]
+    if mode == "persistent":   # one launch per regroup_steps steps at most
+        launches = max(1, dm.last_timing()[1])
     st = dm.fetch("H")
     dm.close()
     return ms, wall, launches, st, tiles
@@ -459,7 +465,7 @@ def main():
 
     host64 = hydro_amd.make_config(WORKLOAD, dtype=np.float64)
     steps = min(a.steps, host64.max_steps)
-    ms, wall, launches, st, tiles = time_gpu(host64, a.mode, steps, a.warmup, a.block, a.tile, a.launch)
+    ms, wall, launches, st, tiles = time_gpu(host64, a.mode, steps, a.warmup, a.block, a.tile, a.launch, a.regroup)
     if not np.all(np.isfinite(st["H"])):
         raise SystemExit("non-finite state after the timed steps")
     C = host64.CELL
@@ -467,7 +473,7 @@ def main():
     per_launch_ms = ms / launches
     two = a.mode in ("two_kernel", "graph_two_kernel")
     steps_per_launch = steps / launches if not two else 1.0
-    bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C * (steps if a.mode == "persistent" else 1)
+    bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C * (steps_per_launch if a.mode == "persistent" else 1)
     if two:
         bytes_per_launch = BYTES_PER_CELL_STEP["f64"] * C / 2.0
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
@@ -503,7 +509,8 @@ def main():
         "config": {"workload": WORKLOAD, "cells": C, "edges": host64.n_edges, "dt": host64.DT,
                    "mode": a.mode, "block": a.block, "tiles": tiles,
                    "tile_cells": "balanced" if a.tile == -1 else (a.tile or a.block), "variant": "A",
-                   "launch": a.launch if resident else None},
+                   "launch": a.launch if resident else None,
+                   "regroup_steps": (200 if a.regroup == 0 else a.regroup) if a.mode != "two_kernel" else None},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell_step": BYTES_PER_CELL_STEP["f64"], "launches": launches,
@@ -512,7 +519,7 @@ def main():
     }
     if not a.no_fp32:
         host32 = hydro_amd.make_config(WORKLOAD, dtype=np.float32)
-        ms32, _, l32, _, _ = time_gpu(host32, a.mode, steps, a.warmup, a.block, a.tile, a.launch)
+        ms32, _, l32, _, _ = time_gpu(host32, a.mode, steps, a.warmup, a.block, a.tile, a.launch, a.regroup)
         v32 = C * steps / (ms32 / 1e3)
         out["fp32"] = {"value": v32, "ms_per_step": ms32 / steps,
                        "roofline_frac": v32 * BYTES_PER_CELL_STEP["f32"] / 1e9 / HBM_PEAK_GBS}

@@ -119,6 +119,8 @@ struct hydro_mesh {
     bool last_resident = false;     // the last PERSISTENT launch ran the resident TILES kernel
     int resident_cap = -1;          // tiles the resident kernel holds at once (-1: not yet asked)
     int flag_base = 0;              // value every tile step flag holds (between persistent TILES launches)
+    int since_group = 0;            // steps since the task plan was last grouped by the state
+    bool defer_check = false;       // later launches of one request: leave the persistent check pending
     int task_stride = 0;
     int n_pairs = 0, n_singles = 0;    // tasks of the plan (statistics)
     DevBuf wbar, adj_flag;   // adjoint scratch: flux-sum adjoints [3][C], unsupported-kind flag
@@ -679,6 +681,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
     HIP_TRY(hipMemset(m->perr.p, 0, sizeof(int)));
     m->graphs.clear();
     m->prepared = true;
+    m->since_group = 0;
     m->int_valid = false;
     return HYDRO_OK;
 }
@@ -703,7 +706,7 @@ template <class R> int ensure_internal(hydro_mesh *m, hipStream_t s) {
 // a partial state: report it at the next call that reads the state or
 // launches again (one synchronous 4-byte read per persistent launch).
 int check_persistent(hydro_mesh *m, hipStream_t s) {
-    if (!m->persist_pending) return HYDRO_OK;
+    if (!m->persist_pending || m->defer_check) return HYDRO_OK;
     m->persist_pending = false;
     int err = 0;
     HIP_TRY(hipStreamSynchronize(s));
@@ -958,6 +961,7 @@ hydro_tune default_tune() {
     t.spin_limit = 0;
     t.persistent_loop = 0;
     t.resident_launch = 0;
+    t.regroup_steps = 0;
     return t;
 }
 
@@ -991,6 +995,93 @@ int check_ptr(const void *p, const char *name, bool required) {
 }
 
 }  // namespace
+
+// Regroup the pair tasks by the current state (on the device, stream-ordered).
+// (A persistent launch that stopped at its spin limit is reported by the
+// next call that steps or reads the state; regrouping a partial state is
+// harmless, so this call does not wait for that check.)
+template <class R> int regroup_t(hydro_mesh *m, hipStream_t s) {
+    const int rc = ensure_internal<R>(m, s);
+    if (rc) return rc;
+    if (m->tune.scheme != HYDRO_SCHEME_TILES) return HYDRO_OK;
+    HIP_TRY(launch_regroup<R>(fused_view<R>(m, m->cur), s));
+    m->since_group = 0;
+    return HYDRO_OK;
+}
+
+// Pair chunks of the plan, and those holding pairs of more than one wet/dry
+// class in the current state (host-side count, synchronous).
+template <class R> int plan_grouping_t(hydro_mesh *m, int *pair_chunks, int *mixed_chunks, hipStream_t s) {
+    int rc = check_persistent(m, s);
+    if (rc || (rc = ensure_internal<R>(m, s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    *pair_chunks = *mixed_chunks = 0;
+    if (m->tune.scheme != HYDRO_SCHEME_TILES) return HYDRO_OK;
+    const int C = m->CELL, B = m->tile, nt = (C + B - 1) / B;
+    std::vector<uint32_t> tw;
+    std::vector<int32_t> tn;
+    std::vector<State<R>> st;
+    std::vector<R> zbc, zb1;
+    if ((rc = d2h(tw, m->task_w.p, m->task_w.bytes / sizeof(uint32_t))) ||
+        (rc = d2h(tn, m->task_n.p, (size_t)nt)) || (rc = d2h(st, m->st[m->cur].p, (size_t)C)) ||
+        (rc = d2h(zbc, m->ZBC.p, (size_t)C)) || (m->need_zb1 && (rc = d2h(zb1, m->ZB1.p, (size_t)C))))
+        return rc;
+    Scalars<R> S;
+    S.HM1 = (R)m->d.HM1;
+    S.HM2 = (R)m->d.HM2;
+    auto wet = [&](int a, int b) {
+        const R zb1a = m->need_zb1 ? zb1[a] : (R)(zbc[a] + S.HM1);
+        return ladder_branch<R>(S, st[a].H, std::max(st[a].Z, zb1a), zbc[a], std::max(st[b].H, S.HM1), zbc[b],
+                                std::max(zbc[b], st[b].Z)) == 7;
+    };
+    for (int t = 0; t < nt; t++)
+        for (int q = 0; q < tn[t]; q++) {
+            int seen = 0, pairs = 0, others = 0;
+            for (int l = 0; l < 64; l++) {
+                const uint32_t w = tw[(size_t)t * m->task_stride + (size_t)q * 64 + l];
+                if (w == TASK_NONE) continue;
+                if (!(w & TASK_PAIR)) {
+                    others++;
+                    continue;
+                }
+                const int a = t * B + (int)task_la(w), b = t * B + (int)task_lb(w);
+                seen |= 1 << (2 - (int)wet(a, b) - (int)wet(b, a));
+                pairs++;
+            }
+            if (pairs == 0 || others > 0) continue;
+            (*pair_chunks)++;
+            if (seen & (seen - 1)) (*mixed_chunks)++;
+        }
+    return HYDRO_OK;
+}
+
+// The reordered modes of the TILES scheme in launches of at most R steps
+// (hydro_tune.regroup_steps, 0 = kRegroupSteps, -1 = no limit), the plan's
+// tasks regrouped by the current state before a launch once R steps have
+// run since the last grouping.
+constexpr int kRegroupSteps = 200;
+template <class R, int VAR> int steps_grouped(hydro_mesh *m, int n, int mode, hipStream_t s) {
+    const int every = m->tune.regroup_steps == 0 ? kRegroupSteps : m->tune.regroup_steps;
+    const bool reordered = mode == HYDRO_MODE_FUSED || mode == HYDRO_MODE_GRAPH || mode == HYDRO_MODE_PERSISTENT;
+    if (every < 0 || !reordered || m->tune.scheme != HYDRO_SCHEME_TILES) return step_dispatch<R, VAR>(m, n, mode, s);
+    int rc = HYDRO_OK;
+    if (n == 0) return step_dispatch<R, VAR>(m, 0, mode, s);
+    // the first launch checks (synchronously) whether the previous persistent
+    // launch stopped early; the later ones of this request do not wait for
+    // each other: a stop is reported by the next call that steps or reads
+    bool first = true;
+    while (n > 0 && rc == HYDRO_OK) {
+        m->defer_check = !first;
+        if (m->prepared && m->since_group >= every) rc = regroup_t<R>(m, s);
+        const int seg = std::min(n, every);
+        if (rc == HYDRO_OK) rc = step_dispatch<R, VAR>(m, seg, mode, s);
+        m->since_group += seg;
+        n -= seg;
+        first = false;
+    }
+    m->defer_check = false;
+    return rc;
+}
 
 extern "C" {
 
@@ -1100,6 +1191,7 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune) {
         return fail(HYDRO_ERR_INVALID, "persistent_loop %d (0 or 1)", t.persistent_loop);
     if (t.resident_launch != 0 && t.resident_launch != 1)
         return fail(HYDRO_ERR_INVALID, "resident_launch %d (0 or 1)", t.resident_launch);
+    if (t.regroup_steps < -1) return fail(HYDRO_ERR_INVALID, "regroup_steps %d (-1, 0 or more)", t.regroup_steps);
     int tile = t.block;
     if (t.scheme == HYDRO_SCHEME_TILES && t.tile_cells > 0) {
         tile = t.tile_cells;
@@ -1191,11 +1283,11 @@ int hydro_step_n(hydro_mesh *m, int nsteps, int mode, hipStream_t stream) {
     }
     int rc;
     if (m->d.dtype == HYDRO_F64)
-        rc = m->d.variant ? step_dispatch<double, 1>(m, nsteps, mode, stream)
-                          : step_dispatch<double, 0>(m, nsteps, mode, stream);
+        rc = m->d.variant ? steps_grouped<double, 1>(m, nsteps, mode, stream)
+                          : steps_grouped<double, 0>(m, nsteps, mode, stream);
     else
-        rc = m->d.variant ? step_dispatch<float, 1>(m, nsteps, mode, stream)
-                          : step_dispatch<float, 0>(m, nsteps, mode, stream);
+        rc = m->d.variant ? steps_grouped<float, 1>(m, nsteps, mode, stream)
+                          : steps_grouped<float, 0>(m, nsteps, mode, stream);
     if (rc) return rc;
     if (m->profiling) {
         HIP_TRY(hipEventRecord(m->ev1, stream));
@@ -1285,6 +1377,15 @@ int hydro_set_cell_coords(hydro_mesh *m, const double *xc, const double *yc) {
     return HYDRO_OK;
 }
 
+int hydro_regroup(hydro_mesh *m, hipStream_t stream) {
+    if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
+    return (m->d.dtype == HYDRO_F64) ? regroup_t<double>(m, stream) : regroup_t<float>(m, stream);
+}
+int hydro_plan_grouping(hydro_mesh *m, int *pair_chunks, int *mixed_chunks, hipStream_t stream) {
+    if (!m || !pair_chunks || !mixed_chunks) return fail(HYDRO_ERR_INVALID, "null argument");
+    return (m->d.dtype == HYDRO_F64) ? plan_grouping_t<double>(m, pair_chunks, mixed_chunks, stream)
+                                     : plan_grouping_t<float>(m, pair_chunks, mixed_chunks, stream);
+}
 int hydro_plan_stats(const hydro_mesh *m, int *tiles, int *pairs, int *singles, double *edges_per_thread) {
     if (!m) return fail(HYDRO_ERR_INVALID, "null mesh");
     if (!m->prepared) return fail(HYDRO_ERR_INVALID, "the mesh is not prepared");

@@ -993,7 +993,124 @@ __global__ void k_refresh_statics(ExtView<R> m, const int32_t *orig, R *ZBC, R *
 
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// Regrouping of each tile's pair tasks by the wet/dry branches their two
+// sides take in the current state (both wet, one side, neither; stable in
+// entry order), as the host planner orders them: the pair tasks of tile t
+// are permuted among the entries that hold pair tasks, so every chunk keeps
+// its place, kind and size.  A pair task reads only its tile's LDS and
+// writes its two slots' own terms, so which entry runs it changes no result.
+// One workgroup of 1024 threads per tile; a tile has at most 2 pairs per
+// cell (two of a cell's four slots each), so at most kMaxPairs.
+constexpr int kMaxPairs = 2 * kTileMaxBlock;
+
+template <class R>
+__device__ __forceinline__ bool regroup_side_wet(const FusedView<R> &f, int a, int b) {
+    const State<R> sa = f.Sin[a], sb = f.Sin[b];
+    const R zba = f.ZBC[a], zbb = f.ZBC[b];
+    const R zb1a = f.ZB1 ? f.ZB1[a] : (R)(zba + f.S.HM1);
+    return ladder_branch<R>(f.S, sa.H, fmax(sa.Z, zb1a), zba, fmax(sb.H, f.S.HM1), zbb, fmax(zbb, sb.Z)) == 7;
+}
+
+template <class R> __global__ void __launch_bounds__(1024) k_regroup(FusedView<R> f) {
+    __shared__ int32_t slot_of[kMaxPairs];   // entry of the r-th pair task in entry order
+    __shared__ int32_t src_of[kMaxPairs];    // entry of the r-th pair task in grouped order
+    __shared__ uint8_t cls_of[kMaxPairs];
+    __shared__ int16_t crank_of[kMaxPairs];  // rank among the pairs of its class, in entry order
+    __shared__ int32_t wcnt[16][4];          // per wavefront: pairs of class 0, 1, 2; all pairs
+    __shared__ int32_t run[4];               // the same over the rounds so far
+    const int t = blockIdx.x;
+    const size_t k0 = (size_t)t * f.task_stride;
+    const int n = 64 * f.task_n[t], p0 = t * f.tile;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (threadIdx.x < 4) run[threadIdx.x] = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int e = base + (int)threadIdx.x;
+        const uint32_t w = e < n ? f.task_w[k0 + e] : TASK_NONE;
+        const bool pair = w != TASK_NONE && (w & TASK_PAIR);
+        int c = 3;
+        if (pair) {
+            const int a = p0 + (int)task_la(w), b = p0 + (int)task_lb(w);
+            c = 2 - (int)regroup_side_wet(f, a, b) - (int)regroup_side_wet(f, b, a);
+        }
+        const uint64_t m0 = __builtin_amdgcn_ballot_w64(pair && c == 0);
+        const uint64_t m1 = __builtin_amdgcn_ballot_w64(pair && c == 1);
+        const uint64_t m2 = __builtin_amdgcn_ballot_w64(pair && c == 2);
+        const uint64_t mp = __builtin_amdgcn_ballot_w64(pair);
+        if (lane == 0) {
+            wcnt[wv][0] = __popcll(m0);
+            wcnt[wv][1] = __popcll(m1);
+            wcnt[wv][2] = __popcll(m2);
+            wcnt[wv][3] = __popcll(mp);
+        }
+        __syncthreads();
+        if (pair) {
+            const uint64_t below = (1ull << lane) - 1;
+            const uint64_t mc = c == 0 ? m0 : c == 1 ? m1 : m2;
+            int rc = run[c] + __popcll(mc & below), rp = run[3] + __popcll(mp & below);
+            for (int v = 0; v < wv; v++) {
+                rc += wcnt[v][c];
+                rp += wcnt[v][3];
+            }
+            if (rp < kMaxPairs) {
+                slot_of[rp] = e;
+                cls_of[rp] = (uint8_t)c;
+                crank_of[rp] = (int16_t)rc;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            int sum = 0;
+            for (int v = 0; v < nw; v++) sum += wcnt[v][threadIdx.x];
+            run[threadIdx.x] += sum;
+        }
+        __syncthreads();
+    }
+    const int np = run[3];
+    if (np > kMaxPairs) return;   // not a plan of this library: left as it is
+    for (int r = threadIdx.x; r < np; r += blockDim.x) {
+        const int c = cls_of[r];
+        src_of[(c == 0 ? 0 : c == 1 ? run[0] : run[0] + run[1]) + crank_of[r]] = slot_of[r];
+    }
+    __syncthreads();
+    // every thread holds at most two moved entries (np <= 2 * blockDim.x):
+    // all are read before any is written
+    uint32_t *tw = const_cast<uint32_t *>(f.task_w);
+    int32_t *tnb = const_cast<int32_t *>(f.task_nb);
+    R *tcs = const_cast<R *>(f.task_cs), *tsd = const_cast<R *>(f.task_sd);
+    uint32_t w0 = 0, w1 = 0;
+    int32_t nb0 = 0, nb1 = 0;
+    R c0 = 0, s0 = 0, d0 = 0, c1 = 0, s1 = 0, d1 = 0;
+    const int r0 = threadIdx.x, r1 = threadIdx.x + blockDim.x;
+    if (r0 < np) {
+        const size_t q = k0 + src_of[r0];
+        w0 = tw[q]; nb0 = tnb[q]; c0 = tcs[2 * q]; s0 = tcs[2 * q + 1]; d0 = tsd[q];
+    }
+    if (r1 < np) {
+        const size_t q = k0 + src_of[r1];
+        w1 = tw[q]; nb1 = tnb[q]; c1 = tcs[2 * q]; s1 = tcs[2 * q + 1]; d1 = tsd[q];
+    }
+    __syncthreads();
+    if (r0 < np) {
+        const size_t q = k0 + slot_of[r0];
+        tw[q] = w0; tnb[q] = nb0; tcs[2 * q] = c0; tcs[2 * q + 1] = s0; tsd[q] = d0;
+    }
+    if (r1 < np) {
+        const size_t q = k0 + slot_of[r1];
+        tw[q] = w1; tnb[q] = nb1; tcs[2 * q] = c1; tcs[2 * q + 1] = s1; tsd[q] = d1;
+    }
+}
+
 }  // namespace
+
+template <class R> hipError_t launch_regroup(const FusedView<R> &f, hipStream_t s) {
+    if (f.scheme != 0 || f.ntiles == 0) return hipSuccess;   // TILES scheme only
+    if (f.tile > kTileMaxBlock) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_regroup<R>), dim3(f.ntiles), dim3(1024), 0, s, f);
+    return hipGetLastError();
+}
+template hipError_t launch_regroup<double>(const FusedView<double> &, hipStream_t);
+template hipError_t launch_regroup<float>(const FusedView<float> &, hipStream_t);
 
 template <class R>
 hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *DTA, R *ZB1,

@@ -175,6 +175,9 @@ template <class R, int VAR>
 hipError_t resident_capacity(int block, bool rare, int *tiles);
 template <class R, int VAR>
 hipError_t launch_resident(const FusedView<R> &f, const PersistentArgs &pa, int remap, bool plain, hipStream_t s);
+// Regroup each tile's pair tasks by the wet/dry branches of the state in
+// f.Sin (TILES scheme; no-op otherwise).
+template <class R> hipError_t launch_regroup(const FusedView<R> &f, hipStream_t s);
 
 // external H U V Z -> internal records (new order)
 template <class R>

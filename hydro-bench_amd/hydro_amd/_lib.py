@@ -13,7 +13,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_void_p
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhydro_amd.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # hydro_status
 OK, ERR_INVALID, ERR_HIP, ERR_RANGE, ERR_UNSUPPORTED, ERR_NOMEM = range(6)
@@ -74,7 +74,7 @@ class Tune(ctypes.Structure):
         ("block", c_int32), ("reorder", c_int32), ("xcd_remap", c_int32), ("graph_chunk", c_int32),
         ("persistent_wgs_per_cu", c_int32), ("materialize_flux", c_int32), ("waves_per_simd", c_int32),
         ("scheme", c_int32), ("tile_cells", c_int32), ("spin_limit", c_int32),
-        ("persistent_loop", c_int32), ("resident_launch", c_int32),
+        ("persistent_loop", c_int32), ("resident_launch", c_int32), ("regroup_steps", c_int32),
     ]
 
 
@@ -105,7 +105,8 @@ def autotune_candidates(hilbert=False):
 
 def default_tune(**kw) -> Tune:
     t = Tune(block=256, reorder=REORDER_TILES, xcd_remap=1, graph_chunk=16, persistent_wgs_per_cu=0,
-             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0, spin_limit=0, persistent_loop=0, resident_launch=0)
+             materialize_flux=0, waves_per_simd=0, scheme=SCHEME_TILES, tile_cells=0, spin_limit=0, persistent_loop=0, resident_launch=0,
+             regroup_steps=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t
@@ -149,6 +150,8 @@ SIGNATURES = {
     "hydro_autotune": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, POINTER(Tune), c_void_p, c_void_p]),
     "hydro_set_cell_coords": (c_int, [c_void_p, c_void_p, c_void_p]),
     "hydro_plan_stats": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_double)]),
+    "hydro_regroup": (c_int, [c_void_p, c_void_p]),
+    "hydro_plan_grouping": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), c_void_p]),
     "hydro_hilbert_order": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "hydro_take_boundary": (c_int, [POINTER(BoundaryDesc), c_char_p] + [c_void_p] * 7),
     "hydro_boundary_increments": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),

@@ -117,6 +117,16 @@ class DeviceMesh:
                                              ctypes.byref(e)))
         return t.value, p.value, s.value, e.value
 
+    def regroup(self):
+        """Regroup the plan's tasks by the current wet/dry state (hydro_regroup)."""
+        _lib.check(self.lib.hydro_regroup(self.h, self.stream()))
+
+    def plan_grouping(self):
+        """(pair chunks, pair chunks mixing wet/dry classes in the current state)."""
+        pc, mc = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.lib.hydro_plan_grouping(self.h, ctypes.byref(pc), ctypes.byref(mc), self.stream()))
+        return pc.value, mc.value
+
     def stream(self):
         return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
 

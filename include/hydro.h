@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define HYDRO_ABI_VERSION 4
+#define HYDRO_ABI_VERSION 5
 
 typedef enum hydro_status {
     HYDRO_OK = 0,
@@ -185,6 +185,11 @@ typedef struct hydro_tune {
                                of another stream or process holds CUs while it runs (two resident
                                launches sharing the GPU could each wait for the other's CUs until
                                the spin limit) */
+    int32_t regroup_steps;  /* TILES scheme, FUSED / GRAPH / PERSISTENT modes: a request is run in
+                               launches (PERSISTENT) or stretches of at most this many steps, and
+                               before one the plan's tasks are regrouped by the current wet/dry state
+                               (hydro_regroup) once this many steps have run since the last grouping;
+                               0 = 200, -1 = never (one launch per request) */
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -216,6 +221,21 @@ int hydro_prepare(hydro_mesh *m, const hydro_tune *tune);
  * cap; synchronises the stream. */
 int hydro_autotune(hydro_mesh *m, int mode, int steps, const hydro_tune *cands, int ncand, hydro_tune *best,
                    float *us_per_step, hipStream_t stream);
+
+/* Regroup the tasks of the prepared plan by the wet/dry branches their edges
+ * take in the current state, on the device (stream-ordered, one small launch;
+ * no counterpart in the reference).  hydro_prepare groups them by the state
+ * it finds; as the wet area moves the chunks mix again and the reordered
+ * modes slow down (~6% over 900 steps of the 207K floodplain), so a long run
+ * calls this every few hundred steps.  Each tile's pair tasks are permuted
+ * among the entries that hold pair tasks; results do not change.  No-op for
+ * the CELL scheme. */
+int hydro_regroup(hydro_mesh *m, hipStream_t stream);
+
+/* Count of the plan's pair chunks (64-entry chunks of pair tasks) and of
+ * those that mix pairs of different wet/dry classes in the current state
+ * (host-side, synchronises the stream; a diagnostic of hydro_regroup). */
+int hydro_plan_grouping(hydro_mesh *m, int *pair_chunks, int *mixed_chunks, hipStream_t stream);
 
 /* Cell centroids (HOST arrays [CELL], caller order) for the HILBERT order;
  * no counterpart in the reference.  Copied; the next hydro_prepare uses them. */
