@@ -94,3 +94,50 @@ def test_regroup_changes_no_result(dtype, mode):
     ref = _run(host, state, 60, "two_kernel")
     for k in FIELDS:
         assert np.array_equal(got[k], ref[k]), k
+
+
+SMALL = ("one cell", "three cells, triangles", "65 cells", "257-cell strip", "triangles and quads", "dry start")
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_frequent_regroups_on_small_meshes(name):
+    """Regrouping every 3 steps (hydro_tune.regroup_steps) on the small and
+    degenerate meshes: every reordered mode equals the two-kernel path bit
+    for bit after 30 steps (across a day boundary)."""
+    import hydro_amd
+    from hydro_amd.device import DeviceMesh
+    from test_gpu_edge_cases import CASES, grid_mesh
+
+    host = grid_mesh(**dict(CASES)[name])
+    ref = DeviceMesh(host, device=0)
+    ref.step(30, "two_kernel")
+    want = ref.fetch(*FIELDS)
+    ref.close()
+    for mode in ("fused", "graph", "persistent"):
+        dm = DeviceMesh(host, device=0, tune=hydro_amd.default_tune(regroup_steps=3))
+        dm.step(30, mode)
+        got = dm.fetch(*FIELDS)
+        dm.close()
+        for k in FIELDS:
+            assert np.array_equal(got[k], want[k]), (mode, k)
+
+
+def test_frequent_regroups_with_every_boundary_kind():
+    """The same on a mesh with every boundary kind (the full kernel variant),
+    breaches opening mid-run."""
+    import hydro_amd
+    from conftest import rare_kinds_mesh
+    from hydro_amd.device import DeviceMesh
+
+    host = rare_kinds_mesh(cached_mesh("channel_6675"), breach_fires=True)
+    ref = DeviceMesh(host, device=0)
+    ref.step(40, "two_kernel")
+    want = ref.fetch(*FIELDS)
+    ref.close()
+    for mode in ("fused", "persistent"):
+        dm = DeviceMesh(host, device=0, tune=hydro_amd.default_tune(regroup_steps=4))
+        dm.step(40, mode)
+        got = dm.fetch(*FIELDS)
+        dm.close()
+        for k in FIELDS:
+            assert np.array_equal(got[k], want[k]), (mode, k)
