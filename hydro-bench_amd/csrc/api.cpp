@@ -1061,8 +1061,12 @@ template <class R> int plan_grouping_t(hydro_mesh *m, int *pair_chunks, int *mix
 // run since the last grouping.
 constexpr int kRegroupSteps = 200;
 template <class R, int VAR> int steps_grouped(hydro_mesh *m, int n, int mode, hipStream_t s) {
-    const int every = m->tune.regroup_steps == 0 ? kRegroupSteps : m->tune.regroup_steps;
+    int every = m->tune.regroup_steps == 0 ? kRegroupSteps : m->tune.regroup_steps;
     const bool reordered = mode == HYDRO_MODE_FUSED || mode == HYDRO_MODE_GRAPH || mode == HYDRO_MODE_PERSISTENT;
+    if (mode == HYDRO_MODE_GRAPH && every > 0) {   // whole replays of steps_graph's chunk per stretch
+        const int chunk = std::max(2, m->tune.graph_chunk + (m->tune.graph_chunk & 1));
+        every = std::max(chunk, every / chunk * chunk);
+    }
     if (every < 0 || !reordered || m->tune.scheme != HYDRO_SCHEME_TILES) return step_dispatch<R, VAR>(m, n, mode, s);
     int rc = HYDRO_OK;
     if (n == 0) return step_dispatch<R, VAR>(m, 0, mode, s);
