@@ -233,6 +233,7 @@ def ensemble_bench(a):
             dm.set_time(0, 1)
             if mode != "two_kernel":
                 dm.step(0, mode)      # gather into the reordered layout outside the timed region
+                dm.regroup()          # the plan grouped for this state, as a fresh prepare leaves it
 
     def sequential(mode):
         for dm in dms:

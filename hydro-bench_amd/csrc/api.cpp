@@ -2081,6 +2081,7 @@ int hydro_refresh_statics(hydro_mesh *m, hipStream_t s) {
 struct hydro_batch {
     std::vector<hydro_mesh *> meshes;
     int dtype = 0, variant = 0, block = 256, remap = 1, chunk = 16;
+    int regroup = 0, since_group = 0;   // hydro_tune.regroup_steps; steps since the members' plans were grouped
     unsigned spin_limit = 1u << 24;
     int ntiles = 0;
     bool rare = false;
@@ -2157,13 +2158,17 @@ void batch_advance(hydro_batch *b, int n) {
     }
 }
 
-template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hipStream_t s) {
+template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hipStream_t s, bool regroup = false) {
     std::vector<char> hv;
     int rc = check_batch_persistent(b, s);
     if (rc) return rc;
     rc = batch_upload<R>(b, s, hv);
     if (rc) return rc;
     const BatchView<R> v = batch_view<R>(b);
+    if (regroup) {
+        HIP_TRY(launch_batch_regroup<R>(v, s));
+        b->since_group = 0;
+    }
     if (mode == HYDRO_MODE_FUSED) {
         for (int i = 0; i < n; i++) HIP_TRY((launch_batch_step<R, VAR>(v, i, b->block, b->remap, s)));
     } else if (mode == HYDRO_MODE_GRAPH) {
@@ -2219,6 +2224,29 @@ template <class R, int VAR> int batch_steps(hydro_batch *b, int n, int mode, hip
     return HYDRO_OK;
 }
 
+// As steps_grouped for a batch: requests in stretches of at most
+// regroup_steps (0 = kRegroupSteps, -1 = one stretch), every member's plan
+// regrouped by its current state before a stretch once that many steps have
+// run since the last grouping.
+template <class R, int VAR> int batch_grouped(hydro_batch *b, int n, int mode, hipStream_t s) {
+    int every = b->regroup == 0 ? kRegroupSteps : b->regroup;
+    if (mode == HYDRO_MODE_GRAPH && every > 0) {
+        const int chunk = std::max(2, b->chunk + (b->chunk & 1));
+        every = std::max(chunk, every / chunk * chunk);
+    }
+    const bool reordered = mode == HYDRO_MODE_FUSED || mode == HYDRO_MODE_GRAPH || mode == HYDRO_MODE_PERSISTENT;
+    if (every < 0 || !reordered || b->scheme != HYDRO_SCHEME_TILES || n == 0)
+        return batch_steps<R, VAR>(b, n, mode, s);
+    while (n > 0) {
+        const int seg = std::min(n, every);
+        const int rc = batch_steps<R, VAR>(b, seg, mode, s, b->since_group >= every);
+        if (rc) return rc;
+        b->since_group += seg;
+        n -= seg;
+    }
+    return HYDRO_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2236,6 +2264,8 @@ int hydro_batch_create(hydro_mesh *const *meshes, int n, const hydro_tune *tune,
     b->remap = t.xcd_remap;
     b->chunk = t.graph_chunk;
     b->scheme = t.scheme;
+    if (t.regroup_steps < -1) return fail(HYDRO_ERR_INVALID, "regroup_steps %d (-1, 0 or more)", t.regroup_steps);
+    b->regroup = t.regroup_steps;
     if (t.spin_limit > 0) b->spin_limit = (unsigned)t.spin_limit;
     if (t.tile_cells <= 0) t.tile_cells = t.block;   // members keep whole tiles: the batch balances them
     std::vector<int32_t> tile_mesh, tile0(n);
@@ -2287,8 +2317,10 @@ int hydro_batch_step_n(hydro_batch *b, int nsteps, int mode, hipStream_t stream)
             return fail(HYDRO_ERR_RANGE, "mesh %zu: %d steps requested, %lld remain", i, nsteps, (long long)rem);
     }
     if (b->dtype == HYDRO_F64)
-        return b->variant ? batch_steps<double, 1>(b, nsteps, mode, stream) : batch_steps<double, 0>(b, nsteps, mode, stream);
-    return b->variant ? batch_steps<float, 1>(b, nsteps, mode, stream) : batch_steps<float, 0>(b, nsteps, mode, stream);
+        return b->variant ? batch_grouped<double, 1>(b, nsteps, mode, stream)
+                          : batch_grouped<double, 0>(b, nsteps, mode, stream);
+    return b->variant ? batch_grouped<float, 1>(b, nsteps, mode, stream)
+                      : batch_grouped<float, 0>(b, nsteps, mode, stream);
 }
 
 int hydro_batch_sync(hydro_batch *b, hipStream_t stream) {

@@ -1011,14 +1011,13 @@ __device__ __forceinline__ bool regroup_side_wet(const FusedView<R> &f, int a, i
     return ladder_branch<R>(f.S, sa.H, fmax(sa.Z, zb1a), zba, fmax(sb.H, f.S.HM1), zbb, fmax(zbb, sb.Z)) == 7;
 }
 
-template <class R> __global__ void __launch_bounds__(1024) k_regroup(FusedView<R> f) {
+template <class R> __device__ void regroup_tile(const FusedView<R> &f, int t) {
     __shared__ int32_t slot_of[kMaxPairs];   // entry of the r-th pair task in entry order
     __shared__ int32_t src_of[kMaxPairs];    // entry of the r-th pair task in grouped order
     __shared__ uint8_t cls_of[kMaxPairs];
     __shared__ int16_t crank_of[kMaxPairs];  // rank among the pairs of its class, in entry order
     __shared__ int32_t wcnt[16][4];          // per wavefront: pairs of class 0, 1, 2; all pairs
     __shared__ int32_t run[4];               // the same over the rounds so far
-    const int t = blockIdx.x;
     const size_t k0 = (size_t)t * f.task_stride;
     const int n = 64 * f.task_n[t], p0 = t * f.tile;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -1101,6 +1100,16 @@ template <class R> __global__ void __launch_bounds__(1024) k_regroup(FusedView<R
     }
 }
 
+template <class R> __global__ void __launch_bounds__(1024) k_regroup(FusedView<R> f) { regroup_tile(f, blockIdx.x); }
+
+// The same for every tile of a batch (each member's view holds its current
+// state in Sin).
+template <class R> __global__ void __launch_bounds__(1024) k_batch_regroup(BatchView<R> b) {
+    const int m = b.tile_mesh[blockIdx.x];
+    const FusedView<R> f = b.views[m];
+    regroup_tile(f, (int)blockIdx.x - b.tile0[m]);
+}
+
 }  // namespace
 
 template <class R> hipError_t launch_regroup(const FusedView<R> &f, hipStream_t s) {
@@ -1111,6 +1120,14 @@ template <class R> hipError_t launch_regroup(const FusedView<R> &f, hipStream_t 
 }
 template hipError_t launch_regroup<double>(const FusedView<double> &, hipStream_t);
 template hipError_t launch_regroup<float>(const FusedView<float> &, hipStream_t);
+
+template <class R> hipError_t launch_batch_regroup(const BatchView<R> &b, hipStream_t s) {
+    if (b.scheme != 0 || b.ntiles == 0) return hipSuccess;   // TILES scheme only
+    hipLaunchKernelGGL((k_batch_regroup<R>), dim3(b.ntiles), dim3(1024), 0, s, b);
+    return hipGetLastError();
+}
+template hipError_t launch_batch_regroup<double>(const BatchView<double> &, hipStream_t);
+template hipError_t launch_batch_regroup<float>(const BatchView<float> &, hipStream_t);
 
 template <class R>
 hipError_t launch_refresh_statics(const ExtView<R> &m, const int32_t *orig, R *ZBC, R *AREA, R *FNC, R *DTA, R *ZB1,

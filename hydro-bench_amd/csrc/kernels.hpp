@@ -212,6 +212,8 @@ hipError_t launch_batch_persistent(const BatchView<R> &b, const PersistentArgs &
                                    hipStream_t s);
 template <class R, int VAR>
 hipError_t batch_persistent_max_grid(int block, bool rare, int scheme, int *grid);
+// launch_regroup for every member of a batch (views holding current states)
+template <class R> hipError_t launch_batch_regroup(const BatchView<R> &b, hipStream_t s);
 
 // internal statics <- the caller's ZBC AREA FNC (and ZB1 when stored); DTA rebuilt from AREA
 template <class R>

@@ -185,11 +185,12 @@ typedef struct hydro_tune {
                                of another stream or process holds CUs while it runs (two resident
                                launches sharing the GPU could each wait for the other's CUs until
                                the spin limit) */
-    int32_t regroup_steps;  /* TILES scheme, FUSED / GRAPH / PERSISTENT modes: a request is run in
-                               launches (PERSISTENT) or stretches of at most this many steps, and
-                               before one the plan's tasks are regrouped by the current wet/dry state
-                               (hydro_regroup) once this many steps have run since the last grouping;
-                               0 = 200, -1 = never (one launch per request) */
+    int32_t regroup_steps;  /* TILES scheme, FUSED / GRAPH / PERSISTENT modes (single meshes and
+                               batches): a request is run in launches (PERSISTENT) or stretches of at
+                               most this many steps (GRAPH: whole replays), and before one the plan's
+                               tasks are regrouped by the current wet/dry state (hydro_regroup; every
+                               member of a batch) once this many steps have run since the last
+                               grouping; 0 = 200, -1 = never (one launch per request) */
 } hydro_tune;
 
 /* ---- lifetime ------------------------------------------------------------ */

@@ -141,3 +141,31 @@ def test_frequent_regroups_with_every_boundary_kind():
         dm.close()
         for k in FIELDS:
             assert np.array_equal(got[k], want[k]), (mode, k)
+
+
+@pytest.mark.parametrize("mode", ["fused", "graph", "persistent"])
+def test_batch_regroups_change_no_result(mode):
+    """A batch regrouping every member every 4 steps equals each member's
+    two-kernel run bit for bit after 30 steps."""
+    import hydro_amd
+    import hydro_amd.mesh as M
+    from hydro_amd.device import DeviceMesh
+    from hydro_amd.ensemble import Batch
+
+    hosts = M.make_ensemble(seed=91, n=4, lo=3000, hi=9000, steps=80, dtype=np.float64, workers=1)
+    tune = hydro_amd.default_tune(block=128)
+    dms = [DeviceMesh(h, device=0, tune=tune) for h in hosts]
+    b = Batch(dms, tune=hydro_amd.default_tune(block=128, regroup_steps=4))
+    b.step(30, mode)
+    b.sync()
+    got = [dm.fetch(*FIELDS) for dm in dms]
+    b.close()
+    for dm in dms:
+        dm.close()
+    for h, g in zip(hosts, got):
+        ref = DeviceMesh(h, device=0)
+        ref.step(30, "two_kernel")
+        want = ref.fetch(*FIELDS)
+        ref.close()
+        for k in FIELDS:
+            assert np.array_equal(g[k], want[k]), k
