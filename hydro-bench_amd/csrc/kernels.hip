@@ -435,25 +435,54 @@ __device__ __forceinline__ State<R> tile_finish(const FusedView<R> &f, const Til
     const int C = f.C;
     const R zb1 = f.ZB1 ? L.zb1[i] : (R)(zbp + S.HM1);
     R WH = 0.0, WU = 0.0, WV = 0.0;
+    if (f.materialize) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        if ((pad >> j) & 1) {
-            Flux<R> fl;
-            if (f.materialize) {
+        for (int j = 0; j < 4; j++) {
+            if ((pad >> j) & 1) {
                 bool fired = false;
-                fl = slot_flux<R, VAR, RARE>(K_MIRROR, S, empty_bc<R>(), me.H, me.U, me.V, me.Z, zb1, zbp, false,
-                                             0, 0, 0, 0, 0, (R)0, (R)0, Flux<R>{0, 0, 0, 0}, &fired);
+                const Flux<R> fl = slot_flux<R, VAR, RARE>(K_MIRROR, S, empty_bc<R>(), me.H, me.U, me.V, me.Z, zb1,
+                                                           zbp, false, 0, 0, 0, 0, 0, (R)0, (R)0,
+                                                           Flux<R>{0, 0, 0, 0}, &fired);
                 const ExtView<R> &m = *f.ext;
                 const int eo = 4 * f.orig[p] + j;
                 m.F0[eo] = fl.f0; m.F1[eo] = fl.f1; m.F2[eo] = fl.f2; m.F3[eo] = fl.f3;
+                accumulate<R>(WH, WU, WV, fl, (R)0, (R)0, (R)0);
             } else {
-                fl = padding_flux_signs<R>(S, me.H, me.U, me.V, me.Z, zb1, zbp);
+                WH += L.c[j * B + i];
+                WU += L.c[(4 + j) * B + i];
+                WV += L.c[(8 + j) * B + i];
             }
-            accumulate<R>(WH, WU, WV, fl, (R)0, (R)0, (R)0);
-        } else {
-            WH += L.c[j * B + i];
-            WU += L.c[(4 + j) * B + i];
-            WV += L.c[(8 + j) * B + i];
+        }
+    } else {
+        // A padding slot adds SIDE * flux = +-0 to each sum.  Adding a zero
+        // leaves every nonzero partial sum as it is, and a zero partial sum
+        // followed by a nonzero term gives that term, so the padding slots
+        // can change a sum only when it comes out exactly zero, and then only
+        // its sign.  The sums are formed without them, and only a cell with
+        // a zero sum forms them again in slot order with the padding slots'
+        // flux signs (the same additions in the same order as the reference).
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (!((pad >> j) & 1)) {
+                WH += L.c[j * B + i];
+                WU += L.c[(4 + j) * B + i];
+                WV += L.c[(8 + j) * B + i];
+            }
+        if ((pad & 15u) && (WH == (R)0 || WU == (R)0 || WV == (R)0)) {
+            const Flux<R> fl = padding_flux_signs<R>(S, me.H, me.U, me.V, me.Z, zb1, zbp);
+            WH = 0.0;
+            WU = 0.0;
+            WV = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if ((pad >> j) & 1) {
+                    accumulate<R>(WH, WU, WV, fl, (R)0, (R)0, (R)0);
+                } else {
+                    WH += L.c[j * B + i];
+                    WU += L.c[(4 + j) * B + i];
+                    WV += L.c[(8 + j) * B + i];
+                }
+            }
         }
     }
     if (f.rec_state) f.rec_state[(SCEN ? (size_t)scen * C : 0) + p] = me;
