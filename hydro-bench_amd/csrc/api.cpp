@@ -278,10 +278,20 @@ template <class R> int prepare_t(hydro_mesh *m) {
     } else {
         return fail(HYDRO_ERR_INVALID, "reorder %d", m->tune.reorder);
     }
+    // the state at planning time (H, Z): cells and tasks are grouped by the
+    // wet/dry branches they take in it
+    std::vector<R> Hs, Zs;
+    if ((rc = d2h(Hs, d.H, C)) || (rc = d2h(Zs, d.Z, C))) return rc;
     // TILES scheme: a tile's cells read by other tiles first, so the
-    // wavefronts holding them finish their update, and publish it, first
-    if (m->tune.scheme == HYDRO_SCHEME_TILES && m->tune.reorder != HYDRO_REORDER_NONE)
-        edge_cells_first(C, NAC.data(), m->tile, perm);
+    // wavefronts holding them finish their update, and publish it, first;
+    // within each group the deep cells first, so the shallow and dry ones
+    // share wavefronts (an update skips the friction terms when every cell
+    // of its wavefront is shallow)
+    if (m->tune.scheme == HYDRO_SCHEME_TILES && m->tune.reorder != HYDRO_REORDER_NONE) {
+        std::vector<uint8_t> shallow(C);
+        for (int o = 0; o < C; o++) shallow[o] = Hs[o] <= (R)d.HM2;
+        edge_cells_first(C, NAC.data(), m->tile, perm, shallow.data());
+    }
     std::vector<int32_t> inv(C);
     for (int p = 0; p < C; p++) inv[perm[p]] = p;
 
@@ -299,10 +309,7 @@ template <class R> int prepare_t(hydro_mesh *m) {
     m->need_zb1 = !zb1_ok;
     m->need_slcs = !slcs_ok;
 
-    // the state at planning time, in the new order (H, Z, ZB1): the pairs of
-    // a tile are grouped by the wet/dry branch they take in it
-    std::vector<R> Hs, Zs;
-    if ((rc = d2h(Hs, d.H, C)) || (rc = d2h(Zs, d.Z, C))) return rc;
+    // the same state in the new order (H, Z, ZB1), for the task grouping
     std::vector<R> h_n(C), z_n(C), zi_n(C);
     for (int p = 0; p < C; p++) {
         const int o = perm[p];

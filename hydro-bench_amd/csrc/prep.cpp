@@ -267,7 +267,7 @@ template void precalculate<float>(int, int, const int32_t *, float *, float *, c
 // share an edge with another tile (a neighbour there, or listed as a
 // neighbour by a cell there) first, the others after them, each group in its
 // previous order.  Tile membership does not change.
-void edge_cells_first(int C, const int32_t *NAC, int B, std::vector<int32_t> &perm) {
+void edge_cells_first(int C, const int32_t *NAC, int B, std::vector<int32_t> &perm, const uint8_t *shallow) {
     if (B <= 0) return;
     std::vector<int32_t> inv(C);
     for (int p = 0; p < C; p++) inv[perm[p]] = p;
@@ -283,6 +283,11 @@ void edge_cells_first(int C, const int32_t *NAC, int B, std::vector<int32_t> &pe
         edge.clear();
         inner.clear();
         for (int p = p0; p < pend; p++) (out[perm[p]] ? edge : inner).push_back(perm[p]);
+        if (shallow) {
+            auto deep_first = [&](int32_t a, int32_t b) { return shallow[a] < shallow[b]; };
+            std::stable_sort(edge.begin(), edge.end(), deep_first);
+            std::stable_sort(inner.begin(), inner.end(), deep_first);
+        }
         std::copy(edge.begin(), edge.end(), perm.begin() + p0);
         std::copy(inner.begin(), inner.end(), perm.begin() + p0 + (int)edge.size());
     }

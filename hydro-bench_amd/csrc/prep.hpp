@@ -21,7 +21,11 @@ std::vector<int32_t> rcm_order(int C, const int32_t *NAC);
 std::vector<int32_t> tile_order(int C, const int32_t *NAC, int B, const std::vector<int32_t> &along);
 // Within each tile of B consecutive positions, cells with a neighbour in
 // another tile first (their updates finish, and are published, first).
-void edge_cells_first(int C, const int32_t *NAC, int B, std::vector<int32_t> &perm);
+// With `shallow` (by original id, nonzero for a cell that is shallow or dry
+// at planning time), each of the two groups lists its deep cells first, so
+// the shallow ones share wavefronts, which then skip the friction terms.
+void edge_cells_first(int C, const int32_t *NAC, int B, std::vector<int32_t> &perm,
+                      const uint8_t *shallow = nullptr);
 
 // Hilbert-curve order of cell centroids (xc, yc), perm[new] = old.
 std::vector<int32_t> hilbert_order(int C, const double *xc, const double *yc);
