@@ -1067,6 +1067,11 @@ template <class R> int plan_grouping_t(hydro_mesh *m, int *pair_chunks, int *mix
 // tasks regrouped by the current state before a launch once R steps have
 // run since the last grouping.
 constexpr int kRegroupSteps = 200;
+// Steps between regroupings for loops that launch step by step (0 = none).
+int regroup_every(const hydro_mesh *m) {
+    if (m->tune.scheme != HYDRO_SCHEME_TILES || m->tune.regroup_steps < 0) return 0;
+    return m->tune.regroup_steps == 0 ? kRegroupSteps : m->tune.regroup_steps;
+}
 template <class R, int VAR> int steps_grouped(hydro_mesh *m, int n, int mode, hipStream_t s) {
     int every = m->tune.regroup_steps == 0 ? kRegroupSteps : m->tune.regroup_steps;
     const bool reordered = mode == HYDRO_MODE_FUSED || mode == HYDRO_MODE_GRAPH || mode == HYDRO_MODE_PERSISTENT;
@@ -1741,8 +1746,10 @@ static int misfit_forward_t(hydro_mesh *m, int nsteps, int seg, void *ckpt, cons
     int rc;
     if ((rc = ensure_internal<R>(m, s)) || (rc = gauge_tables(m, gauges, n, s))) return rc;
     const size_t C = (size_t)m->CELL;
+    const int every = regroup_every(m);
     for (int k = 0; k < nsteps; k++) {
         FusedView<R> f = fused_view<R>(m, m->cur);
+        if (every > 0 && k > 0 && k % every == 0) HIP_TRY(launch_regroup<R>(f, s));   // the plan, by this state
         if (ckpt && k % seg == 0) f.rec_state = (State<R> *)ckpt + (size_t)(k / seg) * C;   // written by the step
         if (n > 0 && k > 0) {   // this step's input is the previous step's output
             f.gauge_map = m->gmap.as<int32_t>();
@@ -1944,9 +1951,12 @@ static int scen_forward_t(hydro_mesh *m, int S, const double *fnc, int nsteps, i
         HIP_TRY(hipMemcpyAsync(m->scen_st[0].as<State<R>>() + k * C, m->st[m->cur].p, m->st[m->cur].bytes,
                                hipMemcpyDeviceToDevice, s));
     const int day0 = m->day, kt0 = m->kt;
+    const int every = regroup_every(m);
     int cur = 0;
     for (int k = 0; k < nsteps; k++) {
         FusedView<R> f = scen_view<R>(m, cur);
+        // the shared plan regrouped by the first scenario's state
+        if (every > 0 && k > 0 && k % every == 0) HIP_TRY(launch_regroup<R>(f, s));
         if (ckpt && k % seg == 0) f.rec_state = (State<R> *)ckpt + (size_t)(k / seg) * rec;   // written by the step
         if (n > 0 && k > 0) {
             f.gauge_map = m->gmap.as<int32_t>();
