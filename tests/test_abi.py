@@ -64,6 +64,9 @@ def test_null_handle_calls_fail_cleanly():
     assert lib.hydro_step_n(None, 1, 0, None) == L.ERR_INVALID
     assert lib.hydro_calculate_flux(None, 1, 0, None) == L.ERR_INVALID
     assert lib.hydro_sync(None, None) == L.ERR_INVALID
+    assert lib.hydro_regroup(None, None) == L.ERR_INVALID
+    pc, mc = ctypes.c_int(), ctypes.c_int()
+    assert lib.hydro_plan_grouping(None, ctypes.byref(pc), ctypes.byref(mc), None) == L.ERR_INVALID
     assert lib.hydro_mesh_destroy(None) == L.OK
 
 
